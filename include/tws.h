@@ -1,0 +1,218 @@
+/*
+ * tws.h -- C ABI of the MI355X-native virtual-pipe water simulation.
+ *
+ * This is the drop-in boundary for the simulation half of terrainwatersim's
+ * Terrain class (terrainwatersim/source/scene/Terrain.h:21-39,75-83) and the
+ * GPU contract it drives: image unit 0 TerrainData rgba32f (r = terrain b,
+ * a = water depth d), image unit 1 Flow rgba32f (outgoing flux through the
+ * +x, -x, +y, -y pipes), image unit 2 FlowMap rg16f (directed flow v), and
+ * the SimulationParameters uniform block {F, K, C}
+ * (terrainwatersim/shader/simulationCommon.glsl:1-13, flowUpdate.comp:5-6,
+ * flowApply.comp:5-7).  Each entry point cites the reference interface it
+ * replaces.
+ *
+ * Conventions
+ *   - Plain C types only; no torch or HIP types.  Streams are passed as void*
+ *     holding a hipStream_t (NULL = a stream the library creates).
+ *   - Every function returns TWS_OK (0) or a negative TWS_ERR_* code; no
+ *     exception crosses the ABI.  tws_last_error() returns a message for the
+ *     calling thread's most recent failure.
+ *   - Host arrays are dense and row-major, x fastest: cell (x, y) is index
+ *     x + y*width, as the reference fills its textures (Terrain.cpp:216-220).
+ *     Flux arrays are 4 floats per cell in the Flow channel order
+ *     (+x, -x, +y, -y); velocity arrays are 2 floats per cell (x, y).
+ *   - A handle is used by one host thread at a time; its work is ordered on
+ *     its stream.  Different handles are independent.
+ */
+#ifndef TWS_H
+#define TWS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TWS_ABI_VERSION 1
+
+/* Status codes. */
+#define TWS_OK 0
+#define TWS_ERR_INVALID (-1)    /* bad argument or handle                   */
+#define TWS_ERR_NOMEM (-2)      /* device or host allocation failed         */
+#define TWS_ERR_DEVICE (-3)     /* a HIP call failed                        */
+#define TWS_ERR_NONFINITE (-4)  /* NaN/Inf seen in state (tws_check_finite) */
+#define TWS_ERR_STATE (-5)      /* call not valid in the handle's state     */
+#define TWS_ERR_CAPACITY (-6)   /* request exceeds a configured limit       */
+
+/* Boundary handling of cells outside the W x H domain. */
+#define TWS_BOUNDARY_CLOSED 0   /* closed walls: no flow leaves or enters    */
+#define TWS_BOUNDARY_ZERO 1     /* the reference shaders' own edge: texels
+                                   outside read as r=g=b=0, a=ghost_alpha and
+                                   water drains into them
+                                   (flowUpdate.comp:14-18, flowApply.comp:16-20) */
+
+/* Per-step constants of the SimulationParameters block
+ * (simulationCommon.glsl:1-13) plus the boundary mode. */
+typedef struct tws_params {
+    float flow_friction;            /* F = FlowFriction_perStep             */
+    float water_acceleration;       /* K = WaterAcceleration_perStep        */
+    float cell_area_inv_timescaled; /* C = CellAreaInv_timeScaled           */
+    int32_t boundary;               /* TWS_BOUNDARY_*                        */
+    float ghost_alpha;              /* alpha read outside the domain in
+                                       TWS_BOUNDARY_ZERO mode (0 for Mesa) */
+} tws_params;
+
+/* Device-resident planes of one simulation (for interop with renderers and
+ * other device code).  Row pitch is in floats.  Valid until the next call
+ * that steps, uploads or destroys the handle. */
+typedef struct tws_device_planes {
+    const float* terrain;   /* b                                  */
+    const float* depth;     /* d                                  */
+    const float* flux[4];   /* +x, -x, +y, -y outgoing flux       */
+    int64_t pitch;
+    int32_t width, height;
+} tws_device_planes;
+
+typedef struct tws_sim tws_sim;
+
+/* ---- parameters --------------------------------------------------------- */
+
+/* F = damping^dt, K = dt*pipe_area*gravity/cell_size, C = dt/cell_size^2:
+ * the physical meaning stated by simulationCommon.glsl:5-12. */
+int tws_params_physical(float dt, float gravity, float cell_size, float pipe_area,
+                        float damping, int32_t boundary, tws_params* out);
+
+/* The reference's own derivation from its UI settings, as
+ * Terrain::SetSimulationStepsPerSecond / SetFlowDamping / SetFlowAcceleration
+ * compute it (Terrain.cpp:175-198; defaults 60 steps/s, damping 0.98,
+ * acceleration 10, world size 1024, resolution 1024: Terrain.cpp:22-30,95). */
+int tws_params_reference(float steps_per_second, float damping, float acceleration,
+                         float grid_world_size, uint32_t grid_resolution, int32_t boundary,
+                         tws_params* out);
+
+/* ---- one simulation ------------------------------------------------------ */
+
+/* Allocates the device state of a width x height grid (replaces the texture
+ * allocation of Terrain::CreateHeightmapFromNoiseAndResetSim,
+ * Terrain.cpp:205,230,237).  State starts at b = d = f = 0. */
+int tws_create(int32_t width, int32_t height, const tws_params* params, void* stream,
+               tws_sim** out);
+int tws_destroy(tws_sim* sim);
+
+/* Replaces Terrain::SetFlowDamping / SetFlowAcceleration /
+ * SetSimulationStepsPerSecond uniform updates (Terrain.cpp:175-198). */
+int tws_set_params(tws_sim* sim, const tws_params* params);
+int tws_get_params(const tws_sim* sim, tws_params* out);
+
+/* Host -> device copy of the state (replaces Texture2D::SetData of the
+ * TerrainData and Flow textures, Terrain.cpp:223,233).  Any pointer may be
+ * NULL: terrain/depth keep their device values, flux resets to zero. */
+int tws_upload(tws_sim* sim, const float* terrain, const float* depth, const float* flux4);
+
+/* Device -> host copy.  Any pointer may be NULL.  velocity2 receives the
+ * directed flow derived from the current flux exactly as flowApply.comp:45-46
+ * computes it (the FlowMap of the last step, without the rg16f rounding). */
+int tws_download(tws_sim* sim, float* depth, float* flux4, float* velocity2);
+
+/* Reference interleaved layout: terrain_rgba32f[W*H*4] is TerrainData
+ * (r = b, g, b channels carried through, a = d) and flow_rgba32f[W*H*4] is
+ * Flow (Terrain.cpp:205-237).  Upload reads r and a; download writes r, a and
+ * g = b = the given fill (0.3 in the reference, Terrain.cpp:217-218).
+ * flowmap_rg16f[W*H*2] receives FlowMap as IEEE half bits, round to nearest
+ * even (flowApply.comp:52).  NULL pointers are skipped. */
+int tws_upload_reference_layout(tws_sim* sim, const float* terrain_rgba32f,
+                                const float* flow_rgba32f);
+int tws_download_reference_layout(tws_sim* sim, float* terrain_rgba32f, float gb_fill,
+                                  float* flow_rgba32f, uint16_t* flowmap_rg16f);
+
+/* One radial brush application, d += saturate(1 - |c - p|^2 / R^2) * I over
+ * the whole domain (replaces Terrain::ApplyRadialWaterBrush, Terrain.cpp:150-168,
+ * and waterBrush.comp:18-32; the caller converts world to texel coordinates). */
+int tws_apply_brush(tws_sim* sim, float center_x, float center_y, float intensity,
+                    float radius_sq);
+
+/* A brush applied before every step of tws_step (a steady source, "rain").
+ * intensity == 0 switches it off. */
+int tws_set_rain(tws_sim* sim, float center_x, float center_y, float intensity,
+                 float radius_sq);
+
+/* Advances n steps (replaces the hot loop of Terrain::PerformSimulationStep,
+ * Terrain.cpp:253-265: per step the flowUpdate then the flowApply dispatch).
+ * Asynchronous on the handle's stream. */
+int tws_step(tws_sim* sim, int32_t n);
+
+/* tws_step with a GPU timer around every kernel launch, the counterpart of the
+ * reference's GL_TIME_ELAPSED query around the simulation block
+ * (Scene.cpp:362-364, TimerQuery.cpp:29-72).  launch_ms[i] receives the
+ * duration of launch i (HIP events on the handle's stream) for the first
+ * `capacity` launches; *launches receives the number of launches.
+ * Synchronises the stream. */
+int tws_step_profiled(tws_sim* sim, int32_t n, float* launch_ms, int32_t capacity,
+                      int32_t* launches);
+
+/* Frame-time driver with the reference's accumulator and 10-step clamp
+ * (Terrain::PerformSimulationStep, Terrain.cpp:240-249).  step_seconds is the
+ * step length; *steps_done (may be NULL) receives the number of steps run. */
+int tws_perform_simulation_step(tws_sim* sim, double frame_seconds, double step_seconds,
+                                int32_t* steps_done);
+
+/* Total water volume sum(d) over the domain, accumulated in FP64 in a fixed
+ * order (deterministic).  Synchronises the stream. */
+int tws_volume(tws_sim* sim, double* out);
+
+/* TWS_OK if every d and f seen by the steps since the last call was finite,
+ * TWS_ERR_NONFINITE otherwise.  Clears the flag.  Synchronises the stream. */
+int tws_check_finite(tws_sim* sim);
+
+int tws_sync(tws_sim* sim);
+int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
+
+/* ---- launch configuration ------------------------------------------------ */
+
+/* Kernel variants.  TWOPASS mirrors the reference's structure (one flux
+ * kernel and one apply kernel per step, neighbours read from memory).  The
+ * TILE variants are the fused, temporally blocked kernel: one launch runs up
+ * to k steps on register/LDS-resident tiles. */
+#define TWS_KERNEL_AUTO 0
+#define TWS_KERNEL_TWOPASS 1
+#define TWS_KERNEL_TILE_128x128 2   /* 1024 threads, 2 columns x 8 rows each */
+#define TWS_KERNEL_TILE_128x64 3    /*  512 threads                          */
+#define TWS_KERNEL_TILE_256x64 4    /* 1024 threads, 4 columns x 4 rows each */
+#define TWS_KERNEL_COUNT 5
+
+/* Selects a kernel variant and the number of steps per launch (1, 2, 4 or 8;
+ * TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the autotuner's table. */
+int tws_set_launch(tws_sim* sim, int32_t kernel, int32_t steps_per_launch);
+int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launch);
+
+/* ---- initial state ---------------------------------------------------------- */
+
+/* Seeded value-noise fBm heightfield into out[width*height] (host memory).
+ * Replaces the terrain half of Terrain::CreateHeightmapFromNoiseAndResetSim
+ * (Terrain.cpp:209-222: NoiseGenerator::GetValueNoise, quintic fade,
+ * period-wrapped lattice, persistence 0.43) with a generator whose lattice is
+ * splitmix64(seed, octave, i, j) instead of the reference's PRNG.  Octave o
+ * has base_frequency*2^o lattice cells across the domain and amplitude
+ * persistence^o; the normalised sum in [0,1] is scaled by height_scale. */
+int tws_generate_fbm_terrain(int32_t width, int32_t height, uint64_t seed, int32_t octaves,
+                             int32_t base_frequency, float persistence, float height_scale,
+                             float* out);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+
+/* Writes, for lanes 0..63 of one wavefront, the lane index read from the
+ * left neighbour lane (out[0..63], -1 where there is none) and from the right
+ * neighbour lane (out[64..127]) by the cross-lane moves of the step kernel. */
+int tws_debug_lane_shift_probe(int32_t* out128);
+
+const char* tws_last_error(void);
+int tws_abi_version(void);
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int tws_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TWS_H */

@@ -1,0 +1,199 @@
+"""terrainwatersim_amd -- MI355X-native virtual-pipe water simulation.
+
+The product is libtws.so (HIP kernels for gfx950 + C ABI, include/tws.h) and
+the C++ host headers over it.  This Python module is a thin ctypes binding
+used by tests and the benchmark; it moves numpy arrays across the ABI and
+does no simulation work itself.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (BOUNDARY_CLOSED, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x64,
+                   KERNEL_TILE_128x128, KERNEL_TILE_256x64, KERNEL_TWOPASS, LIB_PATH,
+                   PROTOTYPES, TILE_KERNELS, DevicePlanes, Params, TwsError, check, lib)
+
+__all__ = [
+    "Simulation", "Params", "TwsError", "params_physical", "params_reference",
+    "generate_fbm_terrain", "device_count", "lane_shift_probe", "lib", "LIB_PATH", "PROTOTYPES",
+    "BOUNDARY_CLOSED", "BOUNDARY_ZERO", "KERNEL_AUTO", "KERNEL_TWOPASS", "KERNEL_TILE_128x128",
+    "KERNEL_TILE_128x64", "KERNEL_TILE_256x64", "TILE_KERNELS",
+]
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a, shape) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def params_physical(dt=0.02, gravity=9.81, cell_size=1.0, pipe_area=1.0, damping=0.995,
+                    boundary=BOUNDARY_CLOSED) -> Params:
+    p = Params()
+    check(lib().tws_params_physical(dt, gravity, cell_size, pipe_area, damping, boundary,
+                                    C.byref(p)))
+    return p
+
+
+def params_reference(steps_per_second=60.0, damping=0.98, acceleration=10.0,
+                     grid_world_size=1024.0, grid_resolution=1024,
+                     boundary=BOUNDARY_CLOSED) -> Params:
+    p = Params()
+    check(lib().tws_params_reference(steps_per_second, damping, acceleration, grid_world_size,
+                                     grid_resolution, boundary, C.byref(p)))
+    return p
+
+
+def params_raw(F: float, K: float, Cc: float, boundary=BOUNDARY_CLOSED,
+               ghost_alpha: float = 0.0) -> Params:
+    p = Params()
+    p.flow_friction, p.water_acceleration, p.cell_area_inv_timescaled = F, K, Cc
+    p.boundary, p.ghost_alpha = boundary, ghost_alpha
+    return p
+
+
+def generate_fbm_terrain(width: int, height: int, seed: int, octaves: int = 6,
+                         base_frequency: int = 4, persistence: float = 0.43,
+                         height_scale: float = 100.0) -> np.ndarray:
+    out = np.empty((height, width), np.float32)
+    check(lib().tws_generate_fbm_terrain(width, height, seed, octaves, base_frequency,
+                                         persistence, height_scale, _ptr(out)))
+    return out
+
+
+def device_count() -> int:
+    return lib().tws_device_count()
+
+
+def lane_shift_probe() -> np.ndarray:
+    out = (C.c_int32 * 128)()
+    check(lib().tws_debug_lane_shift_probe(out))
+    return np.frombuffer(out, dtype=np.int32).copy()
+
+
+class Simulation:
+    """One grid on the device (mirrors the simulation half of the reference's
+    Terrain class, terrainwatersim/source/scene/Terrain.h:21-39,75-83)."""
+
+    def __init__(self, width: int, height: int, params: Params | None = None, stream=None):
+        self.width, self.height = int(width), int(height)
+        self._h = C.c_void_p()
+        p = params if params is not None else params_physical()
+        check(lib().tws_create(self.width, self.height, C.byref(p), stream, C.byref(self._h)))
+
+    # -- lifetime
+    def close(self):
+        if self._h:
+            lib().tws_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state
+    def upload(self, terrain=None, depth=None, flux=None):
+        shp = (self.height, self.width)
+        t = _f32(terrain, shp) if terrain is not None else None
+        d = _f32(depth, shp) if depth is not None else None
+        f = _f32(flux, shp + (4,)) if flux is not None else None
+        check(lib().tws_upload(self._h, _ptr(t), _ptr(d), _ptr(f)))
+
+    def download(self, depth=True, flux=True, velocity=False):
+        shp = (self.height, self.width)
+        d = np.empty(shp, np.float32) if depth else None
+        f = np.empty(shp + (4,), np.float32) if flux else None
+        v = np.empty(shp + (2,), np.float32) if velocity else None
+        check(lib().tws_download(self._h, _ptr(d), _ptr(f), _ptr(v)))
+        return {"depth": d, "flux": f, "velocity": v}
+
+    def upload_reference_layout(self, terrain_rgba=None, flow_rgba=None):
+        shp = (self.height, self.width, 4)
+        t = _f32(terrain_rgba, shp) if terrain_rgba is not None else None
+        f = _f32(flow_rgba, shp) if flow_rgba is not None else None
+        check(lib().tws_upload_reference_layout(self._h, _ptr(t), _ptr(f)))
+
+    def download_reference_layout(self, gb_fill=0.3):
+        shp = (self.height, self.width)
+        t = np.empty(shp + (4,), np.float32)
+        f = np.empty(shp + (4,), np.float32)
+        m = np.empty(shp + (2,), np.uint16)
+        check(lib().tws_download_reference_layout(self._h, _ptr(t), gb_fill, _ptr(f), _ptr(m)))
+        return {"terrain_rgba": t, "flow_rgba": f, "flowmap_rg16f": m}
+
+    # -- parameters and sources
+    def set_params(self, p: Params):
+        check(lib().tws_set_params(self._h, C.byref(p)))
+
+    def get_params(self) -> Params:
+        p = Params()
+        check(lib().tws_get_params(self._h, C.byref(p)))
+        return p
+
+    def apply_brush(self, cx, cy, intensity, radius_sq):
+        check(lib().tws_apply_brush(self._h, cx, cy, intensity, radius_sq))
+
+    def set_rain(self, cx, cy, intensity, radius_sq):
+        check(lib().tws_set_rain(self._h, cx, cy, intensity, radius_sq))
+
+    # -- stepping
+    def step(self, n: int = 1):
+        check(lib().tws_step(self._h, int(n)))
+
+    def step_profiled(self, n: int, capacity: int | None = None) -> np.ndarray:
+        """Steps n and returns the duration (ms) of each kernel launch."""
+        cap = capacity or max(1, n)
+        buf = (C.c_float * cap)()
+        nl = C.c_int32()
+        check(lib().tws_step_profiled(self._h, int(n), buf, cap, C.byref(nl)))
+        return np.frombuffer(buf, dtype=np.float32)[:min(nl.value, cap)].copy()
+
+    def perform_simulation_step(self, frame_seconds: float, step_seconds: float) -> int:
+        n = C.c_int32()
+        check(lib().tws_perform_simulation_step(self._h, frame_seconds, step_seconds, C.byref(n)))
+        return n.value
+
+    def set_launch(self, kernel: int, steps_per_launch: int):
+        check(lib().tws_set_launch(self._h, kernel, steps_per_launch))
+
+    def get_launch(self):
+        k, s = C.c_int32(), C.c_int32()
+        check(lib().tws_get_launch(self._h, C.byref(k), C.byref(s)))
+        return k.value, s.value
+
+    def sync(self):
+        check(lib().tws_sync(self._h))
+
+    def volume(self) -> float:
+        v = C.c_double()
+        check(lib().tws_volume(self._h, C.byref(v)))
+        return v.value
+
+    def check_finite(self) -> bool:
+        code = lib().tws_check_finite(self._h)
+        if code == -4:
+            return False
+        check(code)
+        return True
+
+    def device_planes(self) -> DevicePlanes:
+        p = DevicePlanes()
+        check(lib().tws_get_device_planes(self._h, C.byref(p)))
+        return p

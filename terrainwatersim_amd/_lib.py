@@ -1,0 +1,123 @@
+"""ctypes binding of libtws.so (the C ABI declared in include/tws.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+libtws.so is missing every entry point raises, naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("TWS_LIBRARY", _PKG / "libtws.so"))
+
+TWS_OK = 0
+TWS_ERR_INVALID = -1
+TWS_ERR_NOMEM = -2
+TWS_ERR_DEVICE = -3
+TWS_ERR_NONFINITE = -4
+TWS_ERR_STATE = -5
+TWS_ERR_CAPACITY = -6
+
+BOUNDARY_CLOSED = 0
+BOUNDARY_ZERO = 1
+
+KERNEL_AUTO = 0
+KERNEL_TWOPASS = 1
+KERNEL_TILE_128x128 = 2
+KERNEL_TILE_128x64 = 3
+KERNEL_TILE_256x64 = 4
+TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64)
+
+
+class TwsError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("flow_friction", C.c_float),
+        ("water_acceleration", C.c_float),
+        ("cell_area_inv_timescaled", C.c_float),
+        ("boundary", C.c_int32),
+        ("ghost_alpha", C.c_float),
+    ]
+
+    def as_tuple(self):
+        return (self.flow_friction, self.water_acceleration, self.cell_area_inv_timescaled,
+                self.boundary, self.ghost_alpha)
+
+
+class DevicePlanes(C.Structure):
+    _fields_ = [
+        ("terrain", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("flux", C.c_void_p * 4),
+        ("pitch", C.c_int64),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+    ]
+
+
+_fp = C.POINTER(C.c_float)
+_vp = C.c_void_p
+_i32 = C.c_int32
+
+# name -> (restype, argtypes); every function declared in include/tws.h.
+PROTOTYPES = {
+    "tws_params_physical": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _i32, C.POINTER(Params)]),
+    "tws_params_reference": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, _i32, C.POINTER(Params)]),
+    "tws_create": (C.c_int, [_i32, _i32, C.POINTER(Params), _vp, C.POINTER(_vp)]),
+    "tws_destroy": (C.c_int, [_vp]),
+    "tws_set_params": (C.c_int, [_vp, C.POINTER(Params)]),
+    "tws_get_params": (C.c_int, [_vp, C.POINTER(Params)]),
+    "tws_upload": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "tws_download": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "tws_upload_reference_layout": (C.c_int, [_vp, _vp, _vp]),
+    "tws_download_reference_layout": (C.c_int, [_vp, _vp, C.c_float, _vp, _vp]),
+    "tws_apply_brush": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "tws_set_rain": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "tws_step": (C.c_int, [_vp, _i32]),
+    "tws_step_profiled": (C.c_int, [_vp, _i32, _fp, _i32, C.POINTER(_i32)]),
+    "tws_perform_simulation_step": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(_i32)]),
+    "tws_volume": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "tws_check_finite": (C.c_int, [_vp]),
+    "tws_sync": (C.c_int, [_vp]),
+    "tws_get_device_planes": (C.c_int, [_vp, C.POINTER(DevicePlanes)]),
+    "tws_set_launch": (C.c_int, [_vp, _i32, _i32]),
+    "tws_get_launch": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32)]),
+    "tws_generate_fbm_terrain": (C.c_int, [_i32, _i32, C.c_uint64, _i32, _i32, C.c_float, C.c_float, _vp]),
+    "tws_debug_lane_shift_probe": (C.c_int, [C.POINTER(_i32)]),
+    "tws_last_error": (C.c_char_p, []),
+    "tws_abi_version": (C.c_int, []),
+    "tws_device_count": (C.c_int, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libtws.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m terrainwatersim_amd.build` "
+                "(hipcc, gfx950). There is no CPU fallback.")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(code: int) -> int:
+    if code != TWS_OK:
+        msg = lib().tws_last_error()
+        raise TwsError(code, msg.decode() if msg else "unknown error")
+    return code

@@ -1,0 +1,82 @@
+// tws_internal.hpp -- device-side grid descriptor and kernel launchers shared
+// by the C-ABI front ends (single simulation, batch, server).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tws {
+
+// Terrain height given to ghost cells in closed-wall mode.  Any cell next to
+// such a ghost computes a flux toward it of max(0, f*F + (h - kWallHeight)*K),
+// which is +0 for every |f|, |h| below ~1e28, and the ghost itself emits
+// (BIG * K) * (0 / S) = 0: closed walls, exactly.
+constexpr float kWallHeight = 1.0e30f;
+
+constexpr int kWave = 64;
+
+// One grid as the kernels see it.  All grids of a launch live in one device
+// array; blocks find theirs by tile_begin.
+struct GridDesc {
+    const float* b;          // terrain plane
+    float* d[2];             // depth planes, ping-pong
+    float* f[2][4];          // flux planes (+x,-x,+y,-y), ping-pong
+    int64_t pitch;           // row pitch in floats (multiple of 64)
+    int32_t W, H;
+    int32_t tiles_x, tiles_y;
+    int32_t tile_begin;      // first flattened tile index of this grid
+    int32_t steps;           // steps this launch sequence advances the grid
+    int32_t parity0;         // buffer set holding the state before the sequence
+    int32_t boundary;        // TWS_BOUNDARY_*
+    float F, K, C;
+    float ghost_b, ghost_d, ghost_fw;  // ghost cell: terrain, depth, -y flux
+    float rain_cx, rain_cy, rain_I, rain_R2;
+    int32_t rain_on;
+    int32_t pad_;
+    unsigned int* status;    // bit 0: non-finite value seen
+};
+
+// Tile geometry of the fused kernel.  CPL columns per lane, RPW rows per
+// wave, NW waves, at most KH steps per launch.
+template <int CPL, int RPW, int NW, int KH>
+struct TileGeom {
+    static constexpr int EXW = CPL * kWave;                       // tile width incl. halo
+    static constexpr int EXH = RPW * NW;                          // tile height incl. halo
+    static constexpr int HALOX = ((2 * KH + CPL - 1) / CPL) * CPL;  // keeps vector alignment
+    static constexpr int HALOY = 2 * KH;
+    static constexpr int OX = EXW - 2 * HALOX;                    // output columns per tile
+    static constexpr int OY = EXH - 2 * HALOY;                    // output rows per tile
+    static_assert(OX > 0 && OY > 0, "halo larger than tile");
+};
+
+// Output tile size of a (variant, k) pair; used to lay out tile lists.
+struct TileShape {
+    int ox, oy, threads;
+};
+bool tile_shape(int kernel, int k, TileShape* out);
+
+// Launchers (stream-ordered, return hipError_t).
+hipError_t launch_tile(int kernel, int k, const GridDesc* grids_dev, int ngrids,
+                       int total_tiles, int launch_idx, hipStream_t s);
+hipError_t launch_twopass(const GridDesc* g_dev, const GridDesc& g_host, int parity,
+                          hipStream_t s);
+hipError_t launch_brush(const GridDesc& g, int parity, float cx, float cy, float I, float R2,
+                        hipStream_t s);
+hipError_t launch_velocity(const GridDesc& g, int parity, float* v2, hipStream_t s);
+hipError_t launch_tile_single(int kernel, int k, const GridDesc& g, int launch_idx,
+                              hipStream_t s);
+hipError_t launch_interleave(const GridDesc& g, int parity, float* staging4, hipStream_t s);
+hipError_t launch_deinterleave(const GridDesc& g, int parity, const float* staging4,
+                               hipStream_t s);
+hipError_t launch_ref_unpack(const GridDesc& g, int parity, const float* terrain_rgba,
+                             const float* flow_rgba, float* b_out, hipStream_t s);
+hipError_t launch_ref_pack(const GridDesc& g, int parity, float gb_fill, float* terrain_rgba,
+                           float* flow_rgba, uint16_t* flowmap, hipStream_t s);
+hipError_t launch_volume_partials(const GridDesc& g, int parity, double* partials, int nparts,
+                                  hipStream_t s);
+hipError_t launch_zero_fill(float* p, int64_t n, hipStream_t s);
+hipError_t launch_lane_shift_probe(int* out128, hipStream_t s);
+
+constexpr int kVolumeParts = 512;
+
+}  // namespace tws

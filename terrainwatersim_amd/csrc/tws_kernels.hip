@@ -1,0 +1,648 @@
+// tws_kernels.hip -- gfx950 kernels of the virtual-pipe water step.
+//
+// Hot path: k_tile, the fused and temporally blocked step.  One work group
+// owns an EXW x EXH tile of cells (the output tile plus a halo of 2 cells per
+// step on each side) and advances it up to KH steps without touching HBM:
+//   - each lane keeps CPL adjacent columns x RPW rows of depth d and flux f in
+//     VGPRs for the whole launch; terrain b sits in LDS;
+//   - x-neighbours come from the adjacent lane through DPP wave shifts,
+//     y-neighbours from the lane's own registers, and only the first and last
+//     row of each wave go through LDS;
+//   - per step, two work-group barriers separate the flux phase
+//     (flowUpdate.comp:29-62) from the depth phase (flowApply.comp:32-41).
+// The halo region goes stale by 2 cells per step (flux needs h at radius 1,
+// depth needs flux at radius 1), so after KH steps exactly the interior
+// OX x OY cells are valid and stored.  HBM traffic per launch is one read of
+// b, d, f and one write of d, f per output cell (plus the halo re-read),
+// shared by KH steps.
+//
+// Every expression keeps the shaders' operation order and the library is
+// compiled with -ffp-contract=off, so each cell's result is bit-identical to
+// the FP32 restatement regardless of tile shape or steps per launch.
+
+#include "tws_internal.hpp"
+#include "tws.h"
+
+namespace tws {
+
+// ---------------------------------------------------------------------------
+// Cross-lane moves.  DPP wave_shr:1 (lane i reads lane i-1) and wave_shl:1
+// (lane i reads lane i+1); the lane without a source keeps `fill`.
+__device__ __forceinline__ float from_left_lane(float v, float fill) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right_lane(float v, float fill) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+template <int N>
+__device__ __forceinline__ void vload(const float* p, float (&o)[N]);
+template <>
+__device__ __forceinline__ void vload<1>(const float* p, float (&o)[1]) { o[0] = p[0]; }
+template <>
+__device__ __forceinline__ void vload<2>(const float* p, float (&o)[2]) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    o[0] = v.x; o[1] = v.y;
+}
+template <>
+__device__ __forceinline__ void vload<4>(const float* p, float (&o)[4]) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+template <int N>
+__device__ __forceinline__ void vstore(float* p, const float (&o)[N]);
+template <>
+__device__ __forceinline__ void vstore<1>(float* p, const float (&o)[1]) { p[0] = o[0]; }
+template <>
+__device__ __forceinline__ void vstore<2>(float* p, const float (&o)[2]) {
+    *reinterpret_cast<float2*>(p) = make_float2(o[0], o[1]);
+}
+template <>
+__device__ __forceinline__ void vstore<4>(float* p, const float (&o)[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+__device__ __forceinline__ bool finite4(float a, float b, float c, float d) {
+    return __builtin_isfinite(a) && __builtin_isfinite(b) && __builtin_isfinite(c) &&
+           __builtin_isfinite(d);
+}
+
+// waterBrush.comp:26-28, one texel: saturate(1 - dot(c-p, c-p)/R^2) * I.
+__device__ __forceinline__ float brush_amount(float cx, float cy, float I, float R2, int x, int y) {
+    const float tx = cx - (float)x;
+    const float ty = cy - (float)y;
+    const float dist = tx * tx + ty * ty;
+    const float q = dist / R2;
+    float w = 1.0f - q;
+    w = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
+    return w * I;
+}
+
+// flowUpdate.comp:53-59 for one cell: new outgoing flux from the four head
+// differences, clamped at zero and scaled so the cell cannot drain below 0.
+__device__ __forceinline__ void flux_update(float F, float K, float C, float d, float g0,
+                                            float g1, float g2, float g3, float& f0, float& f1,
+                                            float& f2, float& f3) {
+    float o0 = fmaxf(0.0f, f0 * F + g0 * K);
+    float o1 = fmaxf(0.0f, f1 * F + g1 * K);
+    float o2 = fmaxf(0.0f, f2 * F + g2 * K);
+    float o3 = fmaxf(0.0f, f3 * F + g3 * K);
+    const float total = (((o0 + o1) + o2) + o3) * C;
+    if (total > d) {
+        const float s = d / total;
+        o0 = o0 * s; o1 = o1 * s; o2 = o2 * s; o3 = o3 * s;
+    }
+    f0 = o0; f1 = o1; f2 = o2; f3 = o3;
+}
+
+// ---------------------------------------------------------------------------
+// Fused, temporally blocked step.  Grid: one block per tile of the flattened
+// tile list (all grids of a batch).  g1 is used when ngrids == 0 (a single
+// grid passed by value); otherwise grids[] is searched by tile_begin.
+template <int CPL, int RPW, int NW, int KH>
+__global__ void __launch_bounds__(NW * kWave)
+k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_tiles,
+       int launch_idx) {
+    using G = TileGeom<CPL, RPW, NW, KH>;
+    constexpr int EXW = G::EXW, EXH = G::EXH;
+    __shared__ float s_b[EXH * EXW];
+    __shared__ float s_xh[2][NW][CPL][kWave];  // [0]: first row of a wave, [1]: last row
+    __shared__ float s_xf[2][NW][CPL][kWave];
+
+    // XCD-aware order: the hardware deals blocks to the 8 XCDs round robin;
+    // give each XCD a contiguous run of tiles so neighbouring tiles (whose
+    // halos overlap) share one L2.
+    const int bid = blockIdx.x;
+    const int xcd = bid & 7, idx = bid >> 3;
+    const int q = total_tiles >> 3, rmd = total_tiles & 7;
+    const int tile = xcd * q + (xcd < rmd ? xcd : rmd) + idx;
+
+    GridDesc g;
+    if (ngrids == 0) {
+        g = g1;
+    } else {
+        int lo = 0, hi = ngrids - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (grids[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+        }
+        g = grids[lo];
+    }
+    const int levels = min(KH, g.steps - launch_idx * KH);
+    if (levels <= 0) return;
+    const int parity = (g.parity0 + launch_idx) & 1;
+    const int lt = tile - g.tile_begin;
+    const int ty = lt / g.tiles_x;
+    const int tx = lt - ty * g.tiles_x;
+    const int ox = tx * G::OX - G::HALOX;
+    const int oy = ty * G::OY - G::HALOY;
+    const int W = g.W, H = g.H;
+    const int64_t pitch = g.pitch;
+    const float F = g.F, K = g.K, C = g.C;
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lx = lane * CPL;   // tile-local column of this lane's first cell
+    const int ly = wave * RPW;   // tile-local row of this wave's first row
+    const int x0 = ox + lx;
+
+    float d[RPW][CPL], f0[RPW][CPL], f1[RPW][CPL], f2[RPW][CPL], f3[RPW][CPL];
+    unsigned inside = 0;  // bit r*CPL+c: cell inside the domain
+    bool bad = false;
+
+    // ---- load -------------------------------------------------------------
+    {
+        const float* __restrict__ pb = g.b;
+        const float* __restrict__ pd = parity ? g.d[1] : g.d[0];
+        const float* __restrict__ p0 = parity ? g.f[1][0] : g.f[0][0];
+        const float* __restrict__ p1 = parity ? g.f[1][1] : g.f[0][1];
+        const float* __restrict__ p2 = parity ? g.f[1][2] : g.f[0][2];
+        const float* __restrict__ p3 = parity ? g.f[1][3] : g.f[0][3];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int y = oy + ly + r;
+            const bool vin = (unsigned)y < (unsigned)H && x0 >= 0 && x0 < W;
+            float vb[CPL], vd[CPL], v0[CPL], v1[CPL], v2[CPL], v3[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) vb[c] = vd[c] = v0[c] = v1[c] = v2[c] = v3[c] = 0.0f;
+            if (vin) {
+                const int64_t off = (int64_t)y * pitch + x0;
+                vload<CPL>(pb + off, vb);
+                vload<CPL>(pd + off, vd);
+                vload<CPL>(p0 + off, v0);
+                vload<CPL>(p1 + off, v1);
+                vload<CPL>(p2 + off, v2);
+                vload<CPL>(p3 + off, v3);
+            }
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const bool in = vin && (x0 + c < W);
+                inside |= (in ? 1u : 0u) << (r * CPL + c);
+                s_b[(ly + r) * EXW + lx + c] = in ? vb[c] : g.ghost_b;
+                d[r][c] = in ? vd[c] : g.ghost_d;
+                f0[r][c] = in ? v0[c] : 0.0f;
+                f1[r][c] = in ? v1[c] : 0.0f;
+                f2[r][c] = in ? v2[c] : 0.0f;
+                f3[r][c] = in ? v3[c] : g.ghost_fw;
+                if (in && !(__builtin_isfinite(vd[c]) && finite4(v0[c], v1[c], v2[c], v3[c])))
+                    bad = true;
+            }
+        }
+    }
+    const bool need_mask = g.boundary == TWS_BOUNDARY_ZERO &&
+                           (ox < 0 || oy < 0 || ox + EXW > W || oy + EXH > H);
+    const bool rain = g.rain_on != 0;
+    __syncthreads();  // s_b complete
+
+    for (int lev = 0; lev < levels; ++lev) {
+        if (rain) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (inside & (1u << (r * CPL + c)))
+                        d[r][c] = d[r][c] + brush_amount(g.rain_cx, g.rain_cy, g.rain_I, g.rain_R2,
+                                                         x0 + c, oy + ly + r);
+        }
+
+        // ---- flux phase: flowUpdate.comp:34-62 ------------------------------
+        float h[RPW][CPL];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) h[r][c] = d[r][c] + s_b[(ly + r) * EXW + lx + c];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            s_xh[0][wave][c][lane] = h[0][c];
+            s_xh[1][wave][c][lane] = h[RPW - 1][c];
+        }
+        __syncthreads();
+        float h_up[CPL], h_dn[CPL];  // rows y-1 of the first row and y+1 of the last row
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            h_up[c] = wave > 0 ? s_xh[1][wave - 1][c][lane] : h[0][c];
+            h_dn[c] = wave < NW - 1 ? s_xh[0][wave + 1][c][lane] : h[RPW - 1][c];
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const float h_left = from_left_lane(h[r][CPL - 1], h[r][0]);
+            const float h_right = from_right_lane(h[r][0], h[r][CPL - 1]);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const float own = h[r][c];
+                const float hx1 = c < CPL - 1 ? h[r][c + 1] : h_right;
+                const float hx0 = c > 0 ? h[r][c - 1] : h_left;
+                const float hy1 = r < RPW - 1 ? h[r + 1][c] : h_dn[c];
+                const float hy0 = r > 0 ? h[r - 1][c] : h_up[c];
+                flux_update(F, K, C, d[r][c], own - hx1, own - hx0, own - hy1, own - hy0,
+                            f0[r][c], f1[r][c], f2[r][c], f3[r][c]);
+            }
+        }
+        if (need_mask) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (!(inside & (1u << (r * CPL + c)))) {
+                        f0[r][c] = 0.0f; f1[r][c] = 0.0f; f2[r][c] = 0.0f;
+                        f3[r][c] = g.ghost_fw;
+                    }
+        }
+
+        // ---- depth phase: flowApply.comp:32-41 -------------------------------
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            s_xf[0][wave][c][lane] = f3[0][c];        // -y flux of the first row
+            s_xf[1][wave][c][lane] = f2[RPW - 1][c];  // +y flux of the last row
+        }
+        __syncthreads();
+        float f_up[CPL], f_dn[CPL];  // +y flux of row y-1, -y flux of row y+1
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            f_up[c] = wave > 0 ? s_xf[1][wave - 1][c][lane] : 0.0f;
+            f_dn[c] = wave < NW - 1 ? s_xf[0][wave + 1][c][lane] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const float in_left = from_left_lane(f0[r][CPL - 1], 0.0f);   // X0.x
+            const float in_right = from_right_lane(f1[r][0], 0.0f);      // X1.y
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const float X1y = c < CPL - 1 ? f1[r][c + 1] : in_right;
+                const float X0x = c > 0 ? f0[r][c - 1] : in_left;
+                const float Y1w = r < RPW - 1 ? f3[r + 1][c] : f_dn[c];
+                const float Y0z = r > 0 ? f2[r - 1][c] : f_up[c];
+                const float in = ((X1y + X0x) + Y1w) + Y0z;
+                const float out = ((f0[r][c] + f1[r][c]) + f2[r][c]) + f3[r][c];
+                d[r][c] = fmaxf(0.0f, d[r][c] + (in - out) * C);
+            }
+        }
+        if (need_mask) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (!(inside & (1u << (r * CPL + c)))) d[r][c] = g.ghost_d;
+        }
+    }
+
+    // ---- store the valid interior ------------------------------------------
+    {
+        float* __restrict__ pd = parity ? g.d[0] : g.d[1];
+        float* __restrict__ p0 = parity ? g.f[0][0] : g.f[1][0];
+        float* __restrict__ p1 = parity ? g.f[0][1] : g.f[1][1];
+        float* __restrict__ p2 = parity ? g.f[0][2] : g.f[1][2];
+        float* __restrict__ p3 = parity ? g.f[0][3] : g.f[1][3];
+        const bool col_ok = lx >= G::HALOX && lx < EXW - G::HALOX && x0 < W;
+        const bool full = x0 + CPL <= W;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int yl = ly + r;
+            const int y = oy + yl;
+            if (!(col_ok && yl >= G::HALOY && yl < EXH - G::HALOY && y < H)) continue;
+            const int64_t off = (int64_t)y * pitch + x0;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+                if ((full || x0 + c < W) &&
+                    !(__builtin_isfinite(d[r][c]) && finite4(f0[r][c], f1[r][c], f2[r][c], f3[r][c])))
+                    bad = true;
+            if (full) {
+                vstore<CPL>(pd + off, d[r]);
+                vstore<CPL>(p0 + off, f0[r]);
+                vstore<CPL>(p1 + off, f1[r]);
+                vstore<CPL>(p2 + off, f2[r]);
+                vstore<CPL>(p3 + off, f3[r]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (x0 + c < W) {
+                        pd[off + c] = d[r][c];
+                        p0[off + c] = f0[r][c];
+                        p1[off + c] = f1[r][c];
+                        p2[off + c] = f2[r][c];
+                        p3[off + c] = f3[r][c];
+                    }
+            }
+        }
+    }
+    if (__ballot(bad) != 0 && lane == 0) atomicOr(g.status, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass step with the reference's structure: a flux kernel and an apply
+// kernel per step, neighbours read from memory.  Reads set `parity`, writes
+// flux to set parity^1 (flux pass) and depth to set parity^1 (apply pass).
+__global__ void __launch_bounds__(256) k_flux_twopass(GridDesc g, int parity) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t p = g.pitch, i = (int64_t)y * p + x;
+    const float* b = g.b;
+    const float* d = g.d[parity];
+    const bool in_x1 = x + 1 < g.W, in_x0 = x > 0, in_y1 = y + 1 < g.H, in_y0 = y > 0;
+    const float gh = g.ghost_d + g.ghost_b;  // never used in closed mode
+    const float own = d[i] + b[i];
+    const float hx1 = in_x1 ? d[i + 1] + b[i + 1] : gh;
+    const float hx0 = in_x0 ? d[i - 1] + b[i - 1] : gh;
+    const float hy1 = in_y1 ? d[i + p] + b[i + p] : gh;
+    const float hy0 = in_y0 ? d[i - p] + b[i - p] : gh;
+    float o0 = g.f[parity][0][i], o1 = g.f[parity][1][i], o2 = g.f[parity][2][i],
+          o3 = g.f[parity][3][i];
+    if (g.boundary == TWS_BOUNDARY_CLOSED) {
+        // the flux toward a wall is clamped to zero before the scaling sum
+        float t0 = fmaxf(0.0f, o0 * g.F + (own - hx1) * g.K);
+        float t1 = fmaxf(0.0f, o1 * g.F + (own - hx0) * g.K);
+        float t2 = fmaxf(0.0f, o2 * g.F + (own - hy1) * g.K);
+        float t3 = fmaxf(0.0f, o3 * g.F + (own - hy0) * g.K);
+        if (!in_x1) t0 = 0.0f;
+        if (!in_x0) t1 = 0.0f;
+        if (!in_y1) t2 = 0.0f;
+        if (!in_y0) t3 = 0.0f;
+        const float total = (((t0 + t1) + t2) + t3) * g.C;
+        if (total > d[i]) {
+            const float s = d[i] / total;
+            t0 = t0 * s; t1 = t1 * s; t2 = t2 * s; t3 = t3 * s;
+        }
+        o0 = t0; o1 = t1; o2 = t2; o3 = t3;
+    } else {
+        flux_update(g.F, g.K, g.C, d[i], own - hx1, own - hx0, own - hy1, own - hy0, o0, o1, o2,
+                    o3);
+    }
+    g.f[parity ^ 1][0][i] = o0;
+    g.f[parity ^ 1][1][i] = o1;
+    g.f[parity ^ 1][2][i] = o2;
+    g.f[parity ^ 1][3][i] = o3;
+}
+
+__global__ void __launch_bounds__(256) k_apply_twopass(GridDesc g, int parity) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t p = g.pitch, i = (int64_t)y * p + x;
+    float* const* fo = g.f[parity ^ 1];
+    const float gz = g.boundary == TWS_BOUNDARY_CLOSED ? 0.0f : g.ghost_fw;
+    const float X1y = x + 1 < g.W ? fo[1][i + 1] : 0.0f;
+    const float X0x = x > 0 ? fo[0][i - 1] : 0.0f;
+    const float Y1w = y + 1 < g.H ? fo[3][i + p] : gz;
+    const float Y0z = y > 0 ? fo[2][i - p] : 0.0f;
+    const float in = ((X1y + X0x) + Y1w) + Y0z;
+    const float out = ((fo[0][i] + fo[1][i]) + fo[2][i]) + fo[3][i];
+    g.d[parity ^ 1][i] = fmaxf(0.0f, g.d[parity][i] + (in - out) * g.C);
+}
+
+// One brush application to the depth plane of set `parity`, in place.
+__global__ void __launch_bounds__(256) k_brush(GridDesc g, int parity, float cx, float cy,
+                                                float I, float R2) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t i = (int64_t)y * g.pitch + x;
+    float* d = g.d[parity];
+    d[i] = d[i] + brush_amount(cx, cy, I, R2, x, y);
+}
+
+// Directed flow from the current flux, flowApply.comp:45-46; dense output
+// v2[(x + y*W)*2 + {0,1}].
+__global__ void __launch_bounds__(256) k_velocity(GridDesc g, int parity, float* v2) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t p = g.pitch, i = (int64_t)y * p + x;
+    float* const* f = g.f[parity];
+    const float gz = g.boundary == TWS_BOUNDARY_CLOSED ? 0.0f : g.ghost_fw;
+    const float X1y = x + 1 < g.W ? f[1][i + 1] : 0.0f;
+    const float X0x = x > 0 ? f[0][i - 1] : 0.0f;
+    const float Y1w = y + 1 < g.H ? f[3][i + p] : gz;
+    const float Y0z = y > 0 ? f[2][i - p] : 0.0f;
+    const int64_t o = (int64_t)y * g.W + x;
+    v2[o * 2 + 0] = (X1y - f[0][i]) - (X0x - f[1][i]);
+    v2[o * 2 + 1] = (Y1w - f[2][i]) - (Y0z - f[3][i]);
+}
+
+// Planar flux of set `parity` <-> dense interleaved staging[(x + y*W)*4 + c].
+__global__ void __launch_bounds__(256) k_interleave(GridDesc g, int parity, float* st) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t i = (int64_t)y * g.pitch + x, o = (int64_t)y * g.W + x;
+    *reinterpret_cast<float4*>(st + o * 4) =
+        make_float4(g.f[parity][0][i], g.f[parity][1][i], g.f[parity][2][i], g.f[parity][3][i]);
+}
+__global__ void __launch_bounds__(256) k_deinterleave(GridDesc g, int parity, const float* st) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t i = (int64_t)y * g.pitch + x, o = (int64_t)y * g.W + x;
+    const float4 v = *reinterpret_cast<const float4*>(st + o * 4);
+    g.f[parity][0][i] = v.x; g.f[parity][1][i] = v.y;
+    g.f[parity][2][i] = v.z; g.f[parity][3][i] = v.w;
+}
+
+// Reference interleaved layout (Terrain.cpp:205-237) -> planes.
+__global__ void __launch_bounds__(256) k_ref_unpack(GridDesc g, int parity, const float* terrain,
+                                                     const float* flow, float* b_out) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t i = (int64_t)y * g.pitch + x, o = (int64_t)y * g.W + x;
+    if (terrain) {
+        const float4 t = *reinterpret_cast<const float4*>(terrain + o * 4);
+        b_out[i] = t.x;
+        g.d[parity][i] = t.w;
+    }
+    if (flow) {
+        const float4 v = *reinterpret_cast<const float4*>(flow + o * 4);
+        g.f[parity][0][i] = v.x; g.f[parity][1][i] = v.y;
+        g.f[parity][2][i] = v.z; g.f[parity][3][i] = v.w;
+    }
+}
+
+// Planes -> reference layout; FlowMap as rg16f bits (flowApply.comp:52).
+__global__ void __launch_bounds__(256) k_ref_pack(GridDesc g, int parity, float gb_fill,
+                                                   float* terrain, float* flow, uint16_t* flowmap) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t p = g.pitch, i = (int64_t)y * p + x, o = (int64_t)y * g.W + x;
+    float* const* f = g.f[parity];
+    if (terrain)
+        *reinterpret_cast<float4*>(terrain + o * 4) = make_float4(g.b[i], gb_fill, gb_fill, g.d[parity][i]);
+    if (flow)
+        *reinterpret_cast<float4*>(flow + o * 4) = make_float4(f[0][i], f[1][i], f[2][i], f[3][i]);
+    if (flowmap) {
+        const float gz = g.boundary == TWS_BOUNDARY_CLOSED ? 0.0f : g.ghost_fw;
+        const float X1y = x + 1 < g.W ? f[1][i + 1] : 0.0f;
+        const float X0x = x > 0 ? f[0][i - 1] : 0.0f;
+        const float Y1w = y + 1 < g.H ? f[3][i + p] : gz;
+        const float Y0z = y > 0 ? f[2][i - p] : 0.0f;
+        const _Float16 hx = (_Float16)((X1y - f[0][i]) - (X0x - f[1][i]));
+        const _Float16 hy = (_Float16)((Y1w - f[2][i]) - (Y0z - f[3][i]));
+        flowmap[o * 2 + 0] = __builtin_bit_cast(uint16_t, hx);
+        flowmap[o * 2 + 1] = __builtin_bit_cast(uint16_t, hy);
+    }
+}
+
+// FP64 partial sums of depth: block p takes rows p, p+nparts, ...; threads
+// stride the columns; a fixed tree combines them.  Deterministic.
+__global__ void __launch_bounds__(256) k_volume(GridDesc g, int parity, double* partials,
+                                                 int nparts) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    const float* d = g.d[parity];
+    for (int y = blockIdx.x; y < g.H; y += nparts)
+        for (int x = threadIdx.x; x < g.W; x += 256) acc += (double)d[(int64_t)y * g.pitch + x];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(256) k_zero_fill(float* p, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
+}
+
+// Lane-layout check of the DPP wave shifts used by k_tile.
+__global__ void __launch_bounds__(64) k_lane_shift_probe(int* out) {
+    const int lane = threadIdx.x;
+    out[lane] = __float_as_int(from_left_lane(__int_as_float(lane), __int_as_float(-1)));
+    out[64 + lane] = __float_as_int(from_right_lane(__int_as_float(lane), __int_as_float(-1)));
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers.
+
+static dim3 cell_grid(const GridDesc& g) { return dim3((g.W + 15) / 16, (g.H + 15) / 16); }
+
+template <int CPL, int RPW, int NW>
+static hipError_t launch_tile_k(int k, const GridDesc* grids_dev, int ngrids,
+                                const GridDesc* single, int total_tiles, int launch_idx,
+                                hipStream_t s) {
+    GridDesc g1{};
+    if (single) g1 = *single;
+    const dim3 grid(total_tiles), block(NW * kWave);
+    switch (k) {
+        case 1: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 1>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
+        case 2: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 2>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
+        case 4: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 4>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
+        case 8: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 8>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int CPL, int RPW, int NW>
+static void shape_of(int k, TileShape* out) {
+    int ox = 0, oy = 0;
+    switch (k) {
+        case 1: ox = TileGeom<CPL, RPW, NW, 1>::OX; oy = TileGeom<CPL, RPW, NW, 1>::OY; break;
+        case 2: ox = TileGeom<CPL, RPW, NW, 2>::OX; oy = TileGeom<CPL, RPW, NW, 2>::OY; break;
+        case 4: ox = TileGeom<CPL, RPW, NW, 4>::OX; oy = TileGeom<CPL, RPW, NW, 4>::OY; break;
+        case 8: ox = TileGeom<CPL, RPW, NW, 8>::OX; oy = TileGeom<CPL, RPW, NW, 8>::OY; break;
+    }
+    out->ox = ox; out->oy = oy; out->threads = NW * kWave;
+}
+
+bool tile_shape(int kernel, int k, TileShape* out) {
+    if (k != 1 && k != 2 && k != 4 && k != 8) return false;
+    switch (kernel) {
+        case TWS_KERNEL_TILE_128x128: shape_of<2, 8, 16>(k, out); return true;
+        case TWS_KERNEL_TILE_128x64: shape_of<2, 8, 8>(k, out); return true;
+        case TWS_KERNEL_TILE_256x64: shape_of<4, 4, 16>(k, out); return true;
+        default: return false;
+    }
+}
+
+static hipError_t launch_tile_any(int kernel, int k, const GridDesc* grids_dev, int ngrids,
+                                  const GridDesc* single, int total_tiles, int launch_idx,
+                                  hipStream_t s) {
+    if (total_tiles <= 0) return hipSuccess;
+    switch (kernel) {
+        case TWS_KERNEL_TILE_128x128: return launch_tile_k<2, 8, 16>(k, grids_dev, ngrids, single, total_tiles, launch_idx, s);
+        case TWS_KERNEL_TILE_128x64: return launch_tile_k<2, 8, 8>(k, grids_dev, ngrids, single, total_tiles, launch_idx, s);
+        case TWS_KERNEL_TILE_256x64: return launch_tile_k<4, 4, 16>(k, grids_dev, ngrids, single, total_tiles, launch_idx, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tile(int kernel, int k, const GridDesc* grids_dev, int ngrids, int total_tiles,
+                       int launch_idx, hipStream_t s) {
+    return launch_tile_any(kernel, k, grids_dev, ngrids, nullptr, total_tiles, launch_idx, s);
+}
+
+hipError_t launch_tile_single(int kernel, int k, const GridDesc& g, int launch_idx,
+                              hipStream_t s) {
+    return launch_tile_any(kernel, k, nullptr, 0, &g, g.tiles_x * g.tiles_y, launch_idx, s);
+}
+
+hipError_t launch_twopass(const GridDesc* /*g_dev*/, const GridDesc& g, int parity,
+                          hipStream_t s) {
+    if (g.rain_on)
+        hipLaunchKernelGGL(k_brush, cell_grid(g), dim3(256), 0, s, g, parity, g.rain_cx, g.rain_cy,
+                           g.rain_I, g.rain_R2);
+    hipLaunchKernelGGL(k_flux_twopass, cell_grid(g), dim3(256), 0, s, g, parity);
+    hipLaunchKernelGGL(k_apply_twopass, cell_grid(g), dim3(256), 0, s, g, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_brush(const GridDesc& g, int parity, float cx, float cy, float I, float R2,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_brush, cell_grid(g), dim3(256), 0, s, g, parity, cx, cy, I, R2);
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity(const GridDesc& g, int parity, float* v2, hipStream_t s) {
+    hipLaunchKernelGGL(k_velocity, cell_grid(g), dim3(256), 0, s, g, parity, v2);
+    return hipGetLastError();
+}
+
+hipError_t launch_interleave(const GridDesc& g, int parity, float* staging4, hipStream_t s) {
+    hipLaunchKernelGGL(k_interleave, cell_grid(g), dim3(256), 0, s, g, parity, staging4);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave(const GridDesc& g, int parity, const float* staging4,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_deinterleave, cell_grid(g), dim3(256), 0, s, g, parity, staging4);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_unpack(const GridDesc& g, int parity, const float* terrain_rgba,
+                             const float* flow_rgba, float* b_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_ref_unpack, cell_grid(g), dim3(256), 0, s, g, parity, terrain_rgba,
+                       flow_rgba, b_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_pack(const GridDesc& g, int parity, float gb_fill, float* terrain_rgba,
+                           float* flow_rgba, uint16_t* flowmap, hipStream_t s) {
+    hipLaunchKernelGGL(k_ref_pack, cell_grid(g), dim3(256), 0, s, g, parity, gb_fill, terrain_rgba,
+                       flow_rgba, flowmap);
+    return hipGetLastError();
+}
+
+hipError_t launch_volume_partials(const GridDesc& g, int parity, double* partials, int nparts,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_volume, dim3(nparts), dim3(256), 0, s, g, parity, partials, nparts);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_fill(float* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_lane_shift_probe(int* out128, hipStream_t s) {
+    hipLaunchKernelGGL(k_lane_shift_probe, dim3(1), dim3(64), 0, s, out128);
+    return hipGetLastError();
+}
+
+}  // namespace tws

@@ -1,0 +1,223 @@
+"""Parity of the HIP step (through the C ABI) with the CPU restatement.
+
+Bar: bit-identical d, f and v for the FP32 path (the library and the oracle
+are both built without multiply-add contraction and keep the shaders'
+operation order).  Full-size grids are checked through size-independent
+properties: volume conservation, non-negativity, blocked == unblocked, and a
+bitwise comparison of the first steps against the oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+CFG_PARAMS = dict(dt=0.02, gravity=9.81, cell_size=1.0, pipe_area=1.0, damping=0.995)
+
+
+def random_state(seed, H, W, wet=0.7):
+    rng = np.random.default_rng(seed)
+    b = (rng.random((H, W)) * 5.0).astype(F32)
+    d = (rng.random((H, W)) * 2.0 * (rng.random((H, W)) < wet)).astype(F32)
+    f = (rng.random((H, W, 4)) * 0.5).astype(F32)
+    return b, d, f
+
+
+def run_gpu(tws, b, d, f, n, kernel, k, params, rain=None, velocity=True):
+    H, W = b.shape
+    with tws.Simulation(W, H, params) as s:
+        s.set_launch(kernel, k)
+        s.upload(b, d, f)
+        if rain is not None:
+            s.set_rain(*rain)
+        s.step(n)
+        out = s.download(depth=True, flux=True, velocity=velocity)
+        ok = s.check_finite()
+    assert ok
+    return out
+
+
+def p3(p):
+    return p.flow_friction, p.water_acceleration, p.cell_area_inv_timescaled
+
+
+def assert_same(out, d, f, v=None):
+    assert np.array_equal(out["depth"], d), f"depth differs at {np.argwhere(out['depth'] != d)[:5]}"
+    assert np.array_equal(out["flux"], f), "flux differs"
+    if v is not None:
+        assert np.array_equal(out["velocity"], v), "velocity differs"
+
+
+def test_lane_shift_layout(gpu):
+    out = gpu.lane_shift_probe()
+    assert out[:64].tolist() == [-1] + list(range(63))
+    assert out[64:].tolist() == list(range(1, 64)) + [-1]
+
+
+KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 8), (4, 1), (4, 2), (4, 4), (4, 8)]
+
+
+@pytest.mark.parametrize("kernel,k", KERNELS)
+def test_tile_kernel_bitwise(gpu, oracle, kernel, k):
+    b, d, f = random_state(10 + kernel * 10 + k, 157, 301)
+    p = gpu.params_physical(**CFG_PARAMS)
+    rain = (100.0, 60.0, 1e-3, 256.0)
+    n = 11
+    out = run_gpu(gpu, b, d, f, n, kernel, k, p, rain=rain)
+    de, fe, ve = oracle.step(b, d, f, *p3(p), n=n, rain=rain, velocity=True)
+    assert_same(out, de, fe, ve)
+
+
+def test_twopass_bitwise(gpu, oracle):
+    b, d, f = random_state(3, 90, 70)
+    p = gpu.params_physical(**CFG_PARAMS)
+    out = run_gpu(gpu, b, d, f, 9, gpu.KERNEL_TWOPASS, 1, p, rain=(30.0, 20.0, 1e-2, 100.0))
+    de, fe, ve = oracle.step(b, d, f, *p3(p), n=9, rain=(30.0, 20.0, 1e-2, 100.0), velocity=True)
+    assert_same(out, de, fe, ve)
+
+
+@pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 2), (4, 8)])
+def test_zero_boundary_bitwise(gpu, oracle, kernel, k):
+    b, d, f = random_state(21, 130, 259)
+    p = gpu.params_physical(**CFG_PARAMS, boundary=gpu.BOUNDARY_ZERO)
+    out = run_gpu(gpu, b, d, f, 13, kernel, k, p)
+    de, fe, ve = oracle.step(b, d, f, *p3(p), n=13, boundary=oracle.BOUNDARY_ZERO, velocity=True)
+    assert_same(out, de, fe, ve)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (5, 1), (2, 3), (64, 64), (129, 257)])
+def test_ragged_and_tiny_grids(gpu, oracle, shape):
+    H, W = shape
+    b, d, f = random_state(H * 1000 + W, H, W)
+    p = gpu.params_physical(**CFG_PARAMS)
+    for kernel, k in ((2, 4), (4, 8), (1, 1)):
+        out = run_gpu(gpu, b, d, f, 6, kernel, k, p)
+        de, fe, ve = oracle.step(b, d, f, *p3(p), n=6, velocity=True)
+        assert_same(out, de, fe, ve)
+
+
+def config1_state(tws):
+    b = tws.generate_fbm_terrain(256, 256, seed=1)
+    d = np.zeros((256, 256), F32)
+    return b, d
+
+
+def test_config1_rain_1000_steps(gpu, oracle):
+    """256^2 fBm terrain, rain disc r=16 adding 1e-3 m/step at the centre,
+    closed walls, 1000 steps: bitwise vs the FP32 restatement, volume equal to
+    the FP64 sum of deposits to 1e-6, FP32 vs FP64 error reported."""
+    b, d = config1_state(gpu)
+    p = gpu.params_physical(**CFG_PARAMS)
+    rain = (128.0, 128.0, 1e-3, 256.0)
+    with gpu.Simulation(256, 256, p) as s:
+        s.upload(b, d)
+        s.set_rain(*rain)
+        s.step(1000)
+        out = s.download(velocity=True)
+        vol = s.volume()
+    de, fe, ve = oracle.step(b, d, None, *p3(p), n=1000, rain=rain, velocity=True)
+    assert_same(out, de, fe, ve)
+    dep = float(oracle.brush(np.zeros((256, 256)), *rain, dtype=np.float64).sum()) * 1000
+    assert abs(vol - dep) / dep < 1e-6
+    d64, _, _ = oracle.step(b, d, None, *p3(p), n=1000, rain=rain, dtype=np.float64)
+    m = float(d64.max())
+    rel = np.abs(out["depth"] - d64) / np.maximum(np.abs(d64), 1e-3 * m)
+    print(f"config1 FP32 vs FP64 after 1000 steps: max rel (floor 1e-3*max) {rel.max():.3e}, "
+          f"max abs {np.abs(out['depth'] - d64).max():.3e} m, max depth {m:.4f} m")
+    assert rel.max() < 1e-3
+
+
+def test_full_size_4096_properties(gpu, oracle):
+    """4096^2, uniform 0.5 m sheet on fBm terrain (the HBM-bound workload):
+    first 2 steps bitwise vs the oracle; 1000 steps conserve volume and keep
+    d >= 0; blocked k=4 equals unblocked k=1 after 200 steps."""
+    N = 4096
+    b = gpu.generate_fbm_terrain(N, N, seed=2)
+    d = np.full((N, N), 0.5, F32)
+    p = gpu.params_physical(**CFG_PARAMS)
+    with gpu.Simulation(N, N, p) as s:
+        s.upload(b, d)
+        v0 = s.volume()
+        s.step(2)
+        out = s.download(flux=True)
+    de, fe, _ = oracle.step(b, d, None, *p3(p), n=2)
+    assert_same(out, de, fe)
+    res = {}
+    for kernel, k in ((2, 4), (2, 1)):
+        with gpu.Simulation(N, N, p) as s:
+            s.set_launch(kernel, k)
+            s.upload(b, d)
+            s.step(200)
+            res[k] = s.download(flux=True)
+    assert_same(res[4], res[1]["depth"], res[1]["flux"])
+    with gpu.Simulation(N, N, p) as s:
+        s.upload(b, d)
+        s.step(1000)
+        v1 = s.volume()
+        dd = s.download(flux=False)["depth"]
+        assert s.check_finite()
+    assert (dd >= 0).all()
+    assert abs(v1 - v0) / v0 < 1e-6, (v0, v1)
+
+
+def test_split_run_equals_continuous(gpu):
+    b, d, f = random_state(5, 300, 200)
+    p = gpu.params_physical(**CFG_PARAMS)
+    full = run_gpu(gpu, b, d, f, 37, 2, 4, p, velocity=False)
+    with gpu.Simulation(200, 300, p) as s:
+        s.upload(b, d, f)
+        s.step(20)
+        mid = s.download(flux=True)
+    with gpu.Simulation(200, 300, p) as s:
+        s.upload(b, mid["depth"], mid["flux"])
+        s.step(17)
+        end = s.download(flux=True)
+    assert_same(end, full["depth"], full["flux"])
+
+
+def test_brush_once(gpu, oracle):
+    b, d, f = random_state(8, 100, 120)
+    p = gpu.params_physical(**CFG_PARAMS)
+    with gpu.Simulation(120, 100, p) as s:
+        s.upload(b, d, f)
+        s.apply_brush(33.5, 47.25, 0.02, 32.0)
+        out = s.download(flux=False)
+    assert np.array_equal(out["depth"], oracle.brush(d, 33.5, 47.25, 0.02, 32.0))
+
+
+def test_reference_layout_roundtrip(gpu, oracle):
+    b, d, f = random_state(9, 64, 96)
+    p = gpu.params_physical(**CFG_PARAMS)
+    terrain = np.zeros((64, 96, 4), F32)
+    terrain[..., 0], terrain[..., 1], terrain[..., 2], terrain[..., 3] = b, 0.3, 0.3, d
+    with gpu.Simulation(96, 64, p) as s:
+        s.upload_reference_layout(terrain, f)
+        s.step(5)
+        ref = s.download_reference_layout(gb_fill=0.3)
+        out = s.download(flux=True, velocity=True)
+    de, fe, ve = oracle.step(b, d, f, *p3(p), n=5, velocity=True)
+    assert_same(out, de, fe, ve)
+    assert np.array_equal(ref["terrain_rgba"][..., 0], b)
+    assert np.array_equal(ref["terrain_rgba"][..., 3], de)
+    assert (ref["terrain_rgba"][..., 1:3] == F32(0.3)).all()
+    assert np.array_equal(ref["flow_rgba"], fe)
+    assert np.array_equal(ref["flowmap_rg16f"].view(np.float16), ve.astype(np.float16))
+
+
+def test_nonfinite_input_is_flagged(gpu):
+    b, d, f = random_state(11, 64, 64)
+    d[10, 20] = np.nan
+    p = gpu.params_physical(**CFG_PARAMS)
+    with gpu.Simulation(64, 64, p) as s:
+        s.upload(b, d, f)
+        s.step(3)
+        assert not s.check_finite()
+        assert s.check_finite()  # flag cleared by the read
+
+
+def test_perform_simulation_step_clamps_like_reference(gpu):
+    p = gpu.params_physical(**CFG_PARAMS)
+    with gpu.Simulation(32, 32, p) as s:
+        assert s.perform_simulation_step(0.05, 1.0 / 60.0) == 3
+        assert s.perform_simulation_step(1.0, 1.0 / 60.0) == 10  # clamp; time dropped
+        assert s.perform_simulation_step(0.0, 1.0 / 60.0) == 0

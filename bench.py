@@ -46,7 +46,7 @@ def cpu_baseline(b, d, params, target_s=10.0):
     t0 = time.perf_counter()
     dd, ff, _ = tws_oracle.step(b, d, None, F, K, Cc, n=1)
     one = time.perf_counter() - t0
-    steps = int(max(1, min(100, target_s / max(one, 1e-6))))
+    steps = int(max(1, min(400, target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     tws_oracle.step(b, dd, ff, F, K, Cc, n=steps)
     el = time.perf_counter() - t0
@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--k", type=int, default=0, help="steps per launch (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB STREAM copy kernel (reference for PMC byte counts)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -79,6 +81,11 @@ def main():
         sys.exit(2)
 
     import terrainwatersim_amd as tws
+
+    if args.copy_calibration:
+        gbps = tws.stream_copy_gbps(1 << 30, 10)
+        print(json.dumps({"stream_copy_bytes": 1 << 30, "iterations": 11, "GBps": gbps}))
+        return
 
     N = args.grid
     log(f"bench: generating {N}x{N} fBm terrain (seed 2)")
@@ -113,6 +120,7 @@ def main():
     achieved = bytes_per_launch / (mean_ms * 1e-3)
     finite = sim.check_finite()
     sim.close()
+    copy_gbps = tws.stream_copy_gbps(1 << 30, 20)
 
     traffic = None
     if TRAFFIC_FILE.exists():
@@ -152,6 +160,7 @@ def main():
             "kernel": "k_tile",
             "mean_launch_ms": mean_ms,
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "stream_copy_GBps_measured": copy_gbps,
         },
         "finite": finite,
     }

@@ -119,8 +119,10 @@ int tws_download(tws_sim* sim, float* depth, float* flux4, float* velocity2);
  * (r = b, g, b channels carried through, a = d) and flow_rgba32f[W*H*4] is
  * Flow (Terrain.cpp:205-237).  Upload reads r and a; download writes r, a and
  * g = b = the given fill (0.3 in the reference, Terrain.cpp:217-218).
- * flowmap_rg16f[W*H*2] receives FlowMap as IEEE half bits, round to nearest
- * even (flowApply.comp:52).  NULL pointers are skipped. */
+ * flowmap_rg16f[W*H*2] receives FlowMap as IEEE half bits, rounded toward
+ * zero: the conversion the reference's rg16f image store (flowApply.comp:52)
+ * performs on Mesa, pinned by the llvmpipe fixtures.  NULL pointers are
+ * skipped. */
 int tws_upload_reference_layout(tws_sim* sim, const float* terrain_rgba32f,
                                 const float* flow_rgba32f);
 int tws_download_reference_layout(tws_sim* sim, float* terrain_rgba32f, float gb_fill,
@@ -186,6 +188,10 @@ int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 int tws_set_launch(tws_sim* sim, int32_t kernel, int32_t steps_per_launch);
 int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launch);
 
+/* Host-side float -> IEEE half conversion rounding toward zero, the FlowMap
+ * rounding of tws_download_reference_layout, for n values. */
+int tws_float_to_half_rtz(const float* in, uint16_t* out, int64_t n);
+
 /* ---- initial state ---------------------------------------------------------- */
 
 /* Seeded value-noise fBm heightfield into out[width*height] (host memory).
@@ -201,10 +207,16 @@ int tws_generate_fbm_terrain(int32_t width, int32_t height, uint64_t seed, int32
 
 /* ---- diagnostics ---------------------------------------------------------- */
 
-/* Writes, for lanes 0..63 of one wavefront, the lane index read from the
- * left neighbour lane (out[0..63], -1 where there is none) and from the right
- * neighbour lane (out[64..127]) by the cross-lane moves of the step kernel. */
+/* Lane-layout check of the step kernel's cross-lane moves: lane i of one
+ * wavefront holds i + 1; out[i] receives what lane i reads from its left
+ * neighbour lane and out[64 + i] what it reads from its right neighbour lane
+ * (0 where there is no such lane). */
 int tws_debug_lane_shift_probe(int32_t* out128);
+
+/* Device-to-device STREAM copy of `bytes` (read + write), `iterations`
+ * kernel launches on the default stream; *gbps receives (2 * bytes *
+ * iterations) / time.  The measured HBM peak next to the 8 TB/s spec. */
+int tws_stream_copy_bench(int64_t bytes, int32_t iterations, double* gbps);
 
 const char* tws_last_error(void);
 int tws_abi_version(void);

@@ -91,7 +91,10 @@ int ORACLE_FN(oracle_step)(int W, int H, const ORACLE_REAL* b, ORACLE_REAL* d, O
                     if (!in_y1) o[2] = zero;
                     if (!in_y0) o[3] = zero;
                 }
-                ORACLE_REAL total = (((o[0] + o[1]) + o[2]) + o[3]) * C;         /* :57 */
+                /* :57 "x + y + z + w": Mesa (the executable reference) sums
+                 * the four components pairwise; the llvmpipe fixtures pin
+                 * this order (tests/test_golden.py). */
+                ORACLE_REAL total = ((o[0] + o[1]) + (o[2] + o[3])) * C;         /* :57 */
                 if (total > d[i]) {                                              /* :58 */
                     ORACLE_REAL s = d[i] / total;                                /* :59 */
                     for (int c = 0; c < 4; ++c) o[c] = o[c] * s;

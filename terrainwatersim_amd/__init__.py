@@ -80,6 +80,21 @@ def lane_shift_probe() -> np.ndarray:
     return np.frombuffer(out, dtype=np.int32).copy()
 
 
+def float_to_half_rtz(a) -> np.ndarray:
+    """float32 -> float16 bits, rounded toward zero (the FlowMap rounding)."""
+    x = np.ascontiguousarray(a, np.float32)
+    out = np.empty(x.shape, np.uint16)
+    check(lib().tws_float_to_half_rtz(_ptr(x), _ptr(out), x.size))
+    return out
+
+
+def stream_copy_gbps(nbytes: int = 1 << 30, iterations: int = 20) -> float:
+    """Measured device-to-device copy bandwidth (read + write bytes / s, GB/s)."""
+    g = C.c_double()
+    check(lib().tws_stream_copy_bench(int(nbytes), int(iterations), C.byref(g)))
+    return g.value
+
+
 class Simulation:
     """One grid on the device (mirrors the simulation half of the reference's
     Terrain class, terrainwatersim/source/scene/Terrain.h:21-39,75-83)."""

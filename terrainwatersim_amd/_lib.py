@@ -91,6 +91,8 @@ PROTOTYPES = {
     "tws_get_launch": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32)]),
     "tws_generate_fbm_terrain": (C.c_int, [_i32, _i32, C.c_uint64, _i32, _i32, C.c_float, C.c_float, _vp]),
     "tws_debug_lane_shift_probe": (C.c_int, [C.POINTER(_i32)]),
+    "tws_float_to_half_rtz": (C.c_int, [_vp, _vp, C.c_int64]),
+    "tws_stream_copy_bench": (C.c_int, [C.c_int64, _i32, C.POINTER(C.c_double)]),
     "tws_last_error": (C.c_char_p, []),
     "tws_abi_version": (C.c_int, []),
     "tws_device_count": (C.c_int, []),

@@ -259,6 +259,7 @@ int tws_upload(tws_sim* s, const float* terrain, const float* depth, const float
         TWS_HIP(launch_zero_fill(s->f[s->parity][0], 4 * s->pitch * (int64_t)s->H, s->stream),
                 "tws_upload: clear flux");
     }
+    TWS_HIP(launch_check_finite(g, s->parity, s->stream), "tws_upload: finite check");
     TWS_HIP(hipStreamSynchronize(s->stream), "tws_upload: sync");
     return TWS_OK;
 }
@@ -305,6 +306,7 @@ int tws_upload_reference_layout(tws_sim* s, const float* terrain_rgba32f,
         TWS_HIP(launch_ref_unpack(g, s->parity, nullptr, st, s->b, s->stream),
                 "tws_upload_reference_layout: unpack flow");
     }
+    TWS_HIP(launch_check_finite(g, s->parity, s->stream), "tws_upload_reference_layout: check");
     TWS_HIP(hipStreamSynchronize(s->stream), "tws_upload_reference_layout: sync");
     return TWS_OK;
 }
@@ -527,6 +529,40 @@ int tws_get_launch(const tws_sim* s, int32_t* kernel, int32_t* k) {
     if (!s) return fail(TWS_ERR_INVALID, "tws_get_launch: NULL handle");
     if (kernel) *kernel = s->kernel;
     if (k) *k = s->k;
+    return TWS_OK;
+}
+
+int tws_float_to_half_rtz(const float* in, uint16_t* out, int64_t n) {
+    if ((!in || !out) && n > 0) return fail(TWS_ERR_INVALID, "tws_float_to_half_rtz: NULL");
+    for (int64_t i = 0; i < n; ++i) out[i] = half_rtz(in[i]);
+    return TWS_OK;
+}
+
+int tws_stream_copy_bench(int64_t bytes, int32_t iterations, double* gbps) {
+    if (!gbps || bytes < (1 << 20) || iterations <= 0)
+        return fail(TWS_ERR_INVALID, "tws_stream_copy_bench: invalid argument");
+    bytes &= ~(int64_t)255;
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&src, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemset(src, 0, (size_t)bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_stream_copy(src, dst, bytes, nullptr);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int i = 0; e == hipSuccess && i < iterations; ++i)
+        e = launch_stream_copy(src, dst, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e != hipSuccess) return hip_fail(e, "tws_stream_copy_bench");
+    *gbps = 2.0 * (double)bytes * iterations / (ms * 1e-3) / 1e9;
     return TWS_OK;
 }
 

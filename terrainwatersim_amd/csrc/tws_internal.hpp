@@ -15,6 +15,24 @@ constexpr float kWallHeight = 1.0e30f;
 
 constexpr int kWave = 64;
 
+// float -> IEEE half, rounding toward zero: the conversion the reference's
+// rg16f FlowMap store performs when the shaders run on Mesa
+// (flowApply.comp:52).  Finite values beyond the half range give +-65504.
+__host__ __device__ inline uint16_t half_rtz(float x) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u)  // Inf, NaN (quiet, top payload bits kept)
+        return (uint16_t)(a == 0x7f800000u ? sign | 0x7c00u : sign | 0x7e00u | ((a >> 13) & 0x3ffu));
+    if (a >= 0x47800000u) return (uint16_t)(sign | 0x7bffu);  // |x| >= 65536
+    const int E = (int)(a >> 23);
+    if (E >= 113) return (uint16_t)(sign | ((uint32_t)(E - 112) << 10) | ((a >> 13) & 0x3ffu));
+    if (E == 0) return (uint16_t)sign;  // float denormal: below the half range
+    const int shift = 126 - E;          // >= 14
+    if (shift >= 32) return (uint16_t)sign;
+    return (uint16_t)(sign | (((a & 0x7fffffu) | 0x800000u) >> shift));
+}
+
 // One grid as the kernels see it.  All grids of a launch live in one device
 // array; blocks find theirs by tile_begin.
 struct GridDesc {
@@ -76,6 +94,8 @@ hipError_t launch_volume_partials(const GridDesc& g, int parity, double* partial
                                   hipStream_t s);
 hipError_t launch_zero_fill(float* p, int64_t n, hipStream_t s);
 hipError_t launch_lane_shift_probe(int* out128, hipStream_t s);
+hipError_t launch_check_finite(const GridDesc& g, int parity, hipStream_t s);
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 constexpr int kVolumeParts = 512;
 

@@ -27,46 +27,56 @@ namespace tws {
 
 // ---------------------------------------------------------------------------
 // Cross-lane moves.  DPP wave_shr:1 (lane i reads lane i-1) and wave_shl:1
-// (lane i reads lane i+1); the lane without a source keeps `fill`.
-__device__ __forceinline__ float from_left_lane(float v, float fill) {
-    return __int_as_float(
-        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xF, 0xF, false));
+// (lane i reads lane i+1); with bound_ctrl the lane without a source reads 0.
+__device__ __forceinline__ float from_left_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
-__device__ __forceinline__ float from_right_lane(float v, float fill) {
-    return __int_as_float(
-        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xF, 0xF, false));
+__device__ __forceinline__ float from_right_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 
+// Global-address-space views of the state planes: loads and stores through
+// them are global_* instructions rather than flat_* ones.
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ const gfloat* gptr(const float* p) { return (const gfloat*)p; }
+__device__ __forceinline__ gfloat* gptr(float* p) { return (gfloat*)p; }
+
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) vf2 gfloat2;
+typedef __attribute__((address_space(1))) vf4 gfloat4;
+
 template <int N>
-__device__ __forceinline__ void vload(const float* p, float (&o)[N]);
+__device__ __forceinline__ void vload(const gfloat* p, float (&o)[N]);
 template <>
-__device__ __forceinline__ void vload<1>(const float* p, float (&o)[1]) { o[0] = p[0]; }
+__device__ __forceinline__ void vload<1>(const gfloat* p, float (&o)[1]) { o[0] = p[0]; }
 template <>
-__device__ __forceinline__ void vload<2>(const float* p, float (&o)[2]) {
-    const float2 v = *reinterpret_cast<const float2*>(p);
+__device__ __forceinline__ void vload<2>(const gfloat* p, float (&o)[2]) {
+    const vf2 v = *reinterpret_cast<const gfloat2*>(p);
     o[0] = v.x; o[1] = v.y;
 }
 template <>
-__device__ __forceinline__ void vload<4>(const float* p, float (&o)[4]) {
-    const float4 v = *reinterpret_cast<const float4*>(p);
+__device__ __forceinline__ void vload<4>(const gfloat* p, float (&o)[4]) {
+    const vf4 v = *reinterpret_cast<const gfloat4*>(p);
     o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
 }
 template <int N>
-__device__ __forceinline__ void vstore(float* p, const float (&o)[N]);
+__device__ __forceinline__ void vstore(gfloat* p, const float (&o)[N]);
 template <>
-__device__ __forceinline__ void vstore<1>(float* p, const float (&o)[1]) { p[0] = o[0]; }
+__device__ __forceinline__ void vstore<1>(gfloat* p, const float (&o)[1]) { p[0] = o[0]; }
 template <>
-__device__ __forceinline__ void vstore<2>(float* p, const float (&o)[2]) {
-    *reinterpret_cast<float2*>(p) = make_float2(o[0], o[1]);
+__device__ __forceinline__ void vstore<2>(gfloat* p, const float (&o)[2]) {
+    *reinterpret_cast<gfloat2*>(p) = vf2{o[0], o[1]};
 }
 template <>
-__device__ __forceinline__ void vstore<4>(float* p, const float (&o)[4]) {
-    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+__device__ __forceinline__ void vstore<4>(gfloat* p, const float (&o)[4]) {
+    *reinterpret_cast<gfloat4*>(p) = vf4{o[0], o[1], o[2], o[3]};
 }
 
-__device__ __forceinline__ bool finite4(float a, float b, float c, float d) {
-    return __builtin_isfinite(a) && __builtin_isfinite(b) && __builtin_isfinite(c) &&
-           __builtin_isfinite(d);
+// Non-short-circuit finiteness test (no branches).
+__device__ __forceinline__ bool nonfinite5(float a, float b, float c, float d, float e) {
+    return !__builtin_isfinite(a) | !__builtin_isfinite(b) | !__builtin_isfinite(c) |
+           !__builtin_isfinite(d) | !__builtin_isfinite(e);
 }
 
 // waterBrush.comp:26-28, one texel: saturate(1 - dot(c-p, c-p)/R^2) * I.
@@ -89,9 +99,13 @@ __device__ __forceinline__ void flux_update(float F, float K, float C, float d, 
     float o1 = fmaxf(0.0f, f1 * F + g1 * K);
     float o2 = fmaxf(0.0f, f2 * F + g2 * K);
     float o3 = fmaxf(0.0f, f3 * F + g3 * K);
-    const float total = (((o0 + o1) + o2) + o3) * C;
+    // flowUpdate.comp:57 summed pairwise, the order Mesa executes the reference in
+    const float total = ((o0 + o1) + (o2 + o3)) * C;
     if (total > d) {
-        const float s = d / total;
+        // d == 0 (a dry cell on a slope) gives s = 0/total = +0 exactly, so the
+        // division is only executed where some lane holds water.
+        float s = 0.0f;
+        if (d != 0.0f) s = d / total;
         o0 = o0 * s; o1 = o1 * s; o2 = o2 * s; o3 = o3 * s;
     }
     f0 = o0; f1 = o1; f2 = o2; f3 = o3;
@@ -154,12 +168,12 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
 
     // ---- load -------------------------------------------------------------
     {
-        const float* __restrict__ pb = g.b;
-        const float* __restrict__ pd = parity ? g.d[1] : g.d[0];
-        const float* __restrict__ p0 = parity ? g.f[1][0] : g.f[0][0];
-        const float* __restrict__ p1 = parity ? g.f[1][1] : g.f[0][1];
-        const float* __restrict__ p2 = parity ? g.f[1][2] : g.f[0][2];
-        const float* __restrict__ p3 = parity ? g.f[1][3] : g.f[0][3];
+        const gfloat* __restrict__ pb = gptr(g.b);
+        const gfloat* __restrict__ pd = gptr(parity ? g.d[1] : g.d[0]);
+        const gfloat* __restrict__ p0 = gptr(parity ? g.f[1][0] : g.f[0][0]);
+        const gfloat* __restrict__ p1 = gptr(parity ? g.f[1][1] : g.f[0][1]);
+        const gfloat* __restrict__ p2 = gptr(parity ? g.f[1][2] : g.f[0][2]);
+        const gfloat* __restrict__ p3 = gptr(parity ? g.f[1][3] : g.f[0][3]);
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int y = oy + ly + r;
@@ -186,8 +200,6 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
                 f1[r][c] = in ? v1[c] : 0.0f;
                 f2[r][c] = in ? v2[c] : 0.0f;
                 f3[r][c] = in ? v3[c] : g.ghost_fw;
-                if (in && !(__builtin_isfinite(vd[c]) && finite4(v0[c], v1[c], v2[c], v3[c])))
-                    bad = true;
             }
         }
     }
@@ -227,8 +239,8 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
         }
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            const float h_left = from_left_lane(h[r][CPL - 1], h[r][0]);
-            const float h_right = from_right_lane(h[r][0], h[r][CPL - 1]);
+            const float h_left = from_left_lane(h[r][CPL - 1]);
+            const float h_right = from_right_lane(h[r][0]);
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 const float own = h[r][c];
@@ -266,8 +278,8 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
         }
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            const float in_left = from_left_lane(f0[r][CPL - 1], 0.0f);   // X0.x
-            const float in_right = from_right_lane(f1[r][0], 0.0f);      // X1.y
+            const float in_left = from_left_lane(f0[r][CPL - 1]);   // X0.x
+            const float in_right = from_right_lane(f1[r][0]);      // X1.y
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 const float X1y = c < CPL - 1 ? f1[r][c + 1] : in_right;
@@ -290,11 +302,11 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
 
     // ---- store the valid interior ------------------------------------------
     {
-        float* __restrict__ pd = parity ? g.d[0] : g.d[1];
-        float* __restrict__ p0 = parity ? g.f[0][0] : g.f[1][0];
-        float* __restrict__ p1 = parity ? g.f[0][1] : g.f[1][1];
-        float* __restrict__ p2 = parity ? g.f[0][2] : g.f[1][2];
-        float* __restrict__ p3 = parity ? g.f[0][3] : g.f[1][3];
+        gfloat* __restrict__ pd = gptr(parity ? g.d[0] : g.d[1]);
+        gfloat* __restrict__ p0 = gptr(parity ? g.f[0][0] : g.f[1][0]);
+        gfloat* __restrict__ p1 = gptr(parity ? g.f[0][1] : g.f[1][1]);
+        gfloat* __restrict__ p2 = gptr(parity ? g.f[0][2] : g.f[1][2]);
+        gfloat* __restrict__ p3 = gptr(parity ? g.f[0][3] : g.f[1][3]);
         const bool col_ok = lx >= G::HALOX && lx < EXW - G::HALOX && x0 < W;
         const bool full = x0 + CPL <= W;
 #pragma unroll
@@ -305,9 +317,8 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
             const int64_t off = (int64_t)y * pitch + x0;
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
-                if ((full || x0 + c < W) &&
-                    !(__builtin_isfinite(d[r][c]) && finite4(f0[r][c], f1[r][c], f2[r][c], f3[r][c])))
-                    bad = true;
+                bad |= (full || x0 + c < W) &
+                       nonfinite5(d[r][c], f0[r][c], f1[r][c], f2[r][c], f3[r][c]);
             if (full) {
                 vstore<CPL>(pd + off, d[r]);
                 vstore<CPL>(p0 + off, f0[r]);
@@ -360,7 +371,7 @@ __global__ void __launch_bounds__(256) k_flux_twopass(GridDesc g, int parity) {
         if (!in_x0) t1 = 0.0f;
         if (!in_y1) t2 = 0.0f;
         if (!in_y0) t3 = 0.0f;
-        const float total = (((t0 + t1) + t2) + t3) * g.C;
+        const float total = ((t0 + t1) + (t2 + t3)) * g.C;
         if (total > d[i]) {
             const float s = d[i] / total;
             t0 = t0 * s; t1 = t1 * s; t2 = t2 * s; t3 = t3 * s;
@@ -477,10 +488,8 @@ __global__ void __launch_bounds__(256) k_ref_pack(GridDesc g, int parity, float 
         const float X0x = x > 0 ? f[0][i - 1] : 0.0f;
         const float Y1w = y + 1 < g.H ? f[3][i + p] : gz;
         const float Y0z = y > 0 ? f[2][i - p] : 0.0f;
-        const _Float16 hx = (_Float16)((X1y - f[0][i]) - (X0x - f[1][i]));
-        const _Float16 hy = (_Float16)((Y1w - f[2][i]) - (Y0z - f[3][i]));
-        flowmap[o * 2 + 0] = __builtin_bit_cast(uint16_t, hx);
-        flowmap[o * 2 + 1] = __builtin_bit_cast(uint16_t, hy);
+        flowmap[o * 2 + 0] = half_rtz((X1y - f[0][i]) - (X0x - f[1][i]));
+        flowmap[o * 2 + 1] = half_rtz((Y1w - f[2][i]) - (Y0z - f[3][i]));
     }
 }
 
@@ -507,11 +516,38 @@ __global__ void __launch_bounds__(256) k_zero_fill(float* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
 }
 
+// Finiteness of the state of set `parity` (b, d, f) after an upload: sets
+// bit 0 of *status if any value inside the domain is NaN or Inf.  The step
+// kernel itself only checks what it stores (its max(0, .) clamps would turn
+// a NaN input into 0).
+__global__ void __launch_bounds__(256) k_check_finite(GridDesc g, int parity) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    bool bad = false;
+    if (x < g.W && y < g.H) {
+        const int64_t i = (int64_t)y * g.pitch + x;
+        bad = nonfinite5(g.b[i], g.d[parity][i], g.f[parity][0][i], g.f[parity][1][i],
+                         g.f[parity][2][i]) |
+              !__builtin_isfinite(g.f[parity][3][i]);
+    }
+    if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(g.status, 1u);
+}
+
+// Plain device copy of n float4 (STREAM copy): the measured HBM peak, and
+// the known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src,
+                                                      float4* __restrict__ dst, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
 // Lane-layout check of the DPP wave shifts used by k_tile.
 __global__ void __launch_bounds__(64) k_lane_shift_probe(int* out) {
     const int lane = threadIdx.x;
-    out[lane] = __float_as_int(from_left_lane(__int_as_float(lane), __int_as_float(-1)));
-    out[64 + lane] = __float_as_int(from_right_lane(__int_as_float(lane), __int_as_float(-1)));
+    // lane i holds i + 1, so a lane without a source (which reads 0) shows as 0
+    out[lane] = __float_as_int(from_left_lane(__int_as_float(lane + 1)));
+    out[64 + lane] = __float_as_int(from_right_lane(__int_as_float(lane + 1)));
 }
 
 // ---------------------------------------------------------------------------
@@ -637,6 +673,21 @@ hipError_t launch_zero_fill(float* p, int64_t n, hipStream_t s) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_zero_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_finite(const GridDesc& g, int parity, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_finite, cell_grid(g), dim3(256), 0, s, g, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
+    const int64_t n = bytes / 16;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    if (blocks < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s,
+                       static_cast<const float4*>(src), static_cast<float4*>(dst), n);
     return hipGetLastError();
 }
 

@@ -148,7 +148,7 @@ def _numpy_step(b, d, f, F, K, Cc, n, closed=True):
             t = (f[..., c] * F).astype(F32) + ((h - nb[c]).astype(F32) * K).astype(F32)
             t = np.where(np.isnan(t), F32(0), np.maximum(F32(0), t)).astype(F32)
             o.append(t)
-        tot = ((((o[0] + o[1]).astype(F32) + o[2]).astype(F32) + o[3]).astype(F32) * Cc).astype(F32)
+        tot = (((o[0] + o[1]).astype(F32) + (o[2] + o[3]).astype(F32)).astype(F32) * Cc).astype(F32)
         sc = tot > d
         with np.errstate(divide="ignore", invalid="ignore"):
             s = np.where(sc, d / np.where(sc, tot, F32(1)), F32(1)).astype(F32)

@@ -49,9 +49,10 @@ def assert_same(out, d, f, v=None):
 
 
 def test_lane_shift_layout(gpu):
+    # lane i holds i + 1; a lane with no source lane reads 0
     out = gpu.lane_shift_probe()
-    assert out[:64].tolist() == [-1] + list(range(63))
-    assert out[64:].tolist() == list(range(1, 64)) + [-1]
+    assert out[:64].tolist() == list(range(64))
+    assert out[64:].tolist() == list(range(2, 65)) + [0]
 
 
 KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 8), (4, 1), (4, 2), (4, 4), (4, 8)]

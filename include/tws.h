@@ -183,8 +183,8 @@ int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 #define TWS_KERNEL_TILE_256x64 4    /* 1024 threads, 4 columns x 4 rows each */
 #define TWS_KERNEL_COUNT 5
 
-/* Selects a kernel variant and the number of steps per launch (1, 2, 4 or 8;
- * TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the autotuner's table. */
+/* Selects a kernel variant and the number of steps per launch (1, 2, 3, 4,
+ * 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */
 int tws_set_launch(tws_sim* sim, int32_t kernel, int32_t steps_per_launch);
 int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launch);
 

@@ -397,8 +397,9 @@ static int step_impl(tws_sim* s, int32_t n, hipEvent_t* ev, int ev_pairs, int* l
         // template that holds it, so its halo is no wider than needed
         int kk = s->k;
         if (left < kk) {
-            kk = 1;
-            while (kk < left) kk <<= 1;
+            static const int ladder[] = {1, 2, 3, 4, 6, 8};
+            for (int c : ladder)
+                if (c >= left) { kk = c; break; }
         }
         const int levels = left < kk ? left : kk;
         TileShape sh{};

@@ -27,12 +27,23 @@ namespace tws {
 
 // ---------------------------------------------------------------------------
 // Cross-lane moves.  DPP wave_shr:1 (lane i reads lane i-1) and wave_shl:1
-// (lane i reads lane i+1); with bound_ctrl the lane without a source reads 0.
+// (lane i reads lane i+1).  The lane without a source (0 resp. 63, which sit
+// in the tile halo) reads 0, or `fill` in the *_or variants: heads are filled
+// with the lane's own head so the halo edge sees no gradient and does not
+// send every wave down the flux-scaling division.
 __device__ __forceinline__ float from_left_lane(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_right_lane(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float from_left_lane_or(float v, float fill) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right_lane_or(float v, float fill) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
 
 // Global-address-space views of the state planes: loads and stores through
@@ -239,8 +250,8 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
         }
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            const float h_left = from_left_lane(h[r][CPL - 1]);
-            const float h_right = from_right_lane(h[r][0]);
+            const float h_left = from_left_lane_or(h[r][CPL - 1], h[r][0]);
+            const float h_right = from_right_lane_or(h[r][0], h[r][CPL - 1]);
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 const float own = h[r][c];
@@ -566,6 +577,8 @@ static hipError_t launch_tile_k(int k, const GridDesc* grids_dev, int ngrids,
         case 1: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 1>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
         case 2: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 2>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
         case 4: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 4>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
+        case 3: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 3>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
+        case 6: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 6>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
         case 8: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 8>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
         default: return hipErrorInvalidValue;
     }
@@ -579,13 +592,15 @@ static void shape_of(int k, TileShape* out) {
         case 1: ox = TileGeom<CPL, RPW, NW, 1>::OX; oy = TileGeom<CPL, RPW, NW, 1>::OY; break;
         case 2: ox = TileGeom<CPL, RPW, NW, 2>::OX; oy = TileGeom<CPL, RPW, NW, 2>::OY; break;
         case 4: ox = TileGeom<CPL, RPW, NW, 4>::OX; oy = TileGeom<CPL, RPW, NW, 4>::OY; break;
+        case 3: ox = TileGeom<CPL, RPW, NW, 3>::OX; oy = TileGeom<CPL, RPW, NW, 3>::OY; break;
+        case 6: ox = TileGeom<CPL, RPW, NW, 6>::OX; oy = TileGeom<CPL, RPW, NW, 6>::OY; break;
         case 8: ox = TileGeom<CPL, RPW, NW, 8>::OX; oy = TileGeom<CPL, RPW, NW, 8>::OY; break;
     }
     out->ox = ox; out->oy = oy; out->threads = NW * kWave;
 }
 
 bool tile_shape(int kernel, int k, TileShape* out) {
-    if (k != 1 && k != 2 && k != 4 && k != 8) return false;
+    if (k != 1 && k != 2 && k != 3 && k != 4 && k != 6 && k != 8) return false;
     switch (kernel) {
         case TWS_KERNEL_TILE_128x128: shape_of<2, 8, 16>(k, out); return true;
         case TWS_KERNEL_TILE_128x64: shape_of<2, 8, 8>(k, out); return true;

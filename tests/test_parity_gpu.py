@@ -204,7 +204,8 @@ def test_reference_layout_roundtrip(gpu, oracle):
     assert np.array_equal(ref["terrain_rgba"][..., 3], de)
     assert (ref["terrain_rgba"][..., 1:3] == F32(0.3)).all()
     assert np.array_equal(ref["flow_rgba"], fe)
-    assert np.array_equal(ref["flowmap_rg16f"].view(np.float16), ve.astype(np.float16))
+    # rg16f store rounds toward zero (the Mesa behaviour the fixtures pin)
+    assert np.array_equal(ref["flowmap_rg16f"], gpu.float_to_half_rtz(ve))
 
 
 def test_nonfinite_input_is_flagged(gpu):

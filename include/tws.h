@@ -192,6 +192,153 @@ int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launc
  * rounding of tws_download_reference_layout, for n values. */
 int tws_float_to_half_rtz(const float* in, uint16_t* out, int64_t n);
 
+/* ---- batches: many independent grids advanced together ------------------ */
+/*
+ * No counterpart in the reference (its Terrain owns exactly one grid); this is
+ * the multi-grid form of the same step.  All grids of a batch advance in one
+ * launch sequence per tws_batch_run: the tiles of every grid form one
+ * flattened work list, each grid keeps its own size, constants, rain and
+ * state, and the sequence is captured once as a hipGraph and replayed.  Grid
+ * states are carved from one device allocation of `pool_bytes` made at
+ * creation, so adding and removing grids never allocates device memory; a
+ * grid that does not fit is refused with TWS_ERR_CAPACITY.  Results are
+ * bit-identical to the same grid stepped alone with tws_step.
+ */
+typedef struct tws_batch tws_batch;
+
+typedef struct tws_batch_stats {
+    int32_t grids;          /* grids in the batch                            */
+    int32_t max_grids;
+    int64_t cells;          /* total cells of all grids                      */
+    int64_t pool_bytes;     /* device bytes reserved at creation             */
+    int64_t pool_used;      /* device bytes held by grid states              */
+    int64_t pool_largest_free;
+    int64_t graph_captures; /* launch sequences captured so far              */
+    int64_t graph_replays;  /* captured sequences replayed so far            */
+} tws_batch_stats;
+
+/* Device bytes one grid of width x height takes in a batch (its admission
+ * footprint): 11 FP32 planes (terrain, two depth and eight flux planes) with
+ * rows padded to 256 bytes. */
+int64_t tws_grid_footprint(int32_t width, int32_t height);
+
+int tws_batch_create(int32_t max_grids, int64_t pool_bytes, void* stream, tws_batch** out);
+int tws_batch_destroy(tws_batch* batch);
+/* Adds a grid (state zero) and returns its id; TWS_ERR_CAPACITY if the pool
+ * has no block of tws_grid_footprint() bytes or all max_grids ids are used. */
+int tws_batch_add(tws_batch* batch, int32_t width, int32_t height, const tws_params* params,
+                  int32_t* grid_id);
+int tws_batch_remove(tws_batch* batch, int32_t grid_id);
+int tws_batch_upload(tws_batch* batch, int32_t grid_id, const float* terrain, const float* depth,
+                     const float* flux4);
+int tws_batch_download(tws_batch* batch, int32_t grid_id, float* depth, float* flux4,
+                       float* velocity2);
+int tws_batch_set_params(tws_batch* batch, int32_t grid_id, const tws_params* params);
+int tws_batch_set_rain(tws_batch* batch, int32_t grid_id, float center_x, float center_y,
+                       float intensity, float radius_sq);
+int tws_batch_volume(tws_batch* batch, int32_t grid_id, double* out);
+int tws_batch_check_finite(tws_batch* batch, int32_t grid_id);
+/* Steps grid_id will advance in the next tws_batch_run (0 = stays put). */
+int tws_batch_set_steps(tws_batch* batch, int32_t grid_id, int32_t steps);
+/* Advances every grid by its pending steps in one launch sequence, then
+ * clears the pending counts.  Asynchronous on the batch stream. */
+int tws_batch_run(tws_batch* batch);
+/* tws_batch_set_steps(n) for every grid, then tws_batch_run. */
+int tws_batch_step(tws_batch* batch, int32_t n);
+/* Kernel variant and steps per launch for the whole batch (as tws_set_launch;
+ * TWOPASS is not available for batches). */
+int tws_batch_set_launch(tws_batch* batch, int32_t kernel, int32_t steps_per_launch);
+int tws_batch_get_launch(const tws_batch* batch, int32_t* kernel, int32_t* steps_per_launch);
+/* 1 (default): replay captured hipGraphs; 0: launch kernels directly. */
+int tws_batch_use_graphs(tws_batch* batch, int32_t enable);
+int tws_batch_stats_get(const tws_batch* batch, tws_batch_stats* out);
+int tws_batch_sync(tws_batch* batch);
+
+/* ---- serving: a request loop over a batch --------------------------------- */
+/*
+ * No counterpart in the reference.  A server owns a batch and a thread.
+ * Submitted solves (a grid, its initial state, a step count) wait in a queue;
+ * the thread admits them in arrival order while their footprint
+ * (tws_grid_footprint) fits the pool, keeps admitted state on the device,
+ * and advances every resident request by min(remaining, chunk_steps) per
+ * batched run, so requests that arrive while others run join at the next
+ * chunk (continuous batching).  A finished request's outputs are copied to
+ * the caller's buffers and its block returns to the pool.  Each result is
+ * bit-identical to the same solve run alone with tws_step.
+ * Input arrays are copied at submit; output buffers must stay valid until
+ * the request is DONE, FAILED or CANCELLED.
+ */
+typedef struct tws_server tws_server;
+
+typedef struct tws_server_config {
+    int64_t pool_bytes;        /* device bytes for resident request state     */
+    int32_t max_resident;      /* requests on the device at once              */
+    int32_t chunk_steps;       /* steps per batched run (admission latency)   */
+    int32_t kernel;            /* TWS_KERNEL_* for the batch (0 = auto)       */
+    int32_t steps_per_launch;  /* 0 = auto                                    */
+    int32_t max_queue;         /* queued requests before submit refuses       */
+} tws_server_config;
+
+typedef struct tws_request {
+    int32_t width, height;
+    int32_t steps;
+    tws_params params;
+    const float* terrain;      /* width*height, required                      */
+    const float* depth;        /* width*height, NULL = dry                    */
+    const float* flux4;        /* width*height*4, NULL = zero                 */
+    int32_t rain_on;           /* brush before every step when nonzero        */
+    float rain[4];             /* centre x, centre y, intensity, radius^2     */
+    float* out_depth;          /* outputs written when the request completes; */
+    float* out_flux4;          /* any may be NULL                             */
+    float* out_velocity2;
+} tws_request;
+
+#define TWS_REQ_QUEUED 0
+#define TWS_REQ_RUNNING 1
+#define TWS_REQ_DONE 2
+#define TWS_REQ_FAILED 3
+#define TWS_REQ_CANCELLED 4
+
+typedef struct tws_request_status {
+    int32_t state;             /* TWS_REQ_*                                   */
+    int32_t error;             /* TWS_ERR_* when FAILED (NONFINITE: outputs
+                                  were still written)                         */
+    int32_t steps_done;
+    int32_t pad_;
+    double t_submit;           /* seconds on the server's monotonic clock     */
+    double t_admit;
+    double t_done;
+} tws_request_status;
+
+typedef struct tws_server_stats {
+    int64_t submitted, completed, failed, cancelled;
+    int64_t queued, resident;
+    int64_t runs;              /* batched runs executed                       */
+    int64_t cell_updates;      /* cell-updates completed                      */
+    double busy_seconds;       /* wall time spent in batched runs             */
+    int64_t pool_used, pool_bytes;
+} tws_server_stats;
+
+int tws_server_create(const tws_server_config* config, tws_server** out);
+/* Queues a request; *ticket identifies it.  TWS_ERR_CAPACITY if the queue is
+ * full or the grid can never fit the pool. */
+int tws_server_submit(tws_server* server, const tws_request* request, uint64_t* ticket);
+int tws_server_poll(tws_server* server, uint64_t ticket, tws_request_status* out);
+/* Waits until the request leaves QUEUED/RUNNING or timeout_seconds pass
+ * (TWS_ERR_STATE on timeout). */
+int tws_server_wait(tws_server* server, uint64_t ticket, double timeout_seconds,
+                    tws_request_status* out);
+/* Cancels a queued or running request (it leaves the device at the next
+ * chunk boundary; its outputs are not written). */
+int tws_server_cancel(tws_server* server, uint64_t ticket);
+/* Blocks until every submitted request has finished. */
+int tws_server_drain(tws_server* server);
+int tws_server_stats_get(tws_server* server, tws_server_stats* out);
+/* Server monotonic clock, in the time base of tws_request_status. */
+double tws_server_now(const tws_server* server);
+/* Stops the thread (cancelling what is left) and frees everything. */
+int tws_server_destroy(tws_server* server);
+
 /* ---- initial state ---------------------------------------------------------- */
 
 /* Seeded value-noise fBm heightfield into out[width*height] (host memory).

@@ -13,10 +13,12 @@ import numpy as np
 
 from ._lib import (BOUNDARY_CLOSED, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x64,
                    KERNEL_TILE_128x128, KERNEL_TILE_256x64, KERNEL_TWOPASS, LIB_PATH,
-                   PROTOTYPES, TILE_KERNELS, DevicePlanes, Params, TwsError, check, lib)
+                   PROTOTYPES, REQ_CANCELLED, REQ_DONE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING,
+                   TILE_KERNELS, TWS_ERR_CAPACITY, BatchStats, DevicePlanes, Params, Request,
+                   RequestStatus, ServerConfig, ServerStats, TwsError, check, lib)
 
 __all__ = [
-    "Simulation", "Params", "TwsError", "params_physical", "params_reference",
+    "Simulation", "Batch", "Server", "Params", "TwsError", "grid_footprint", "params_physical", "params_reference",
     "generate_fbm_terrain", "device_count", "lane_shift_probe", "lib", "LIB_PATH", "PROTOTYPES",
     "BOUNDARY_CLOSED", "BOUNDARY_ZERO", "KERNEL_AUTO", "KERNEL_TWOPASS", "KERNEL_TILE_128x128",
     "KERNEL_TILE_128x64", "KERNEL_TILE_256x64", "TILE_KERNELS",
@@ -212,3 +214,196 @@ class Simulation:
         p = DevicePlanes()
         check(lib().tws_get_device_planes(self._h, C.byref(p)))
         return p
+
+
+def grid_footprint(width: int, height: int) -> int:
+    """Device bytes of one grid's state (its admission footprint in a batch)."""
+    return lib().tws_grid_footprint(int(width), int(height))
+
+
+class Batch:
+    """Many independent grids advanced by one launch sequence (tws_batch_*)."""
+
+    def __init__(self, max_grids: int, pool_bytes: int, stream=None):
+        self._h = C.c_void_p()
+        self.shapes = {}
+        check(lib().tws_batch_create(int(max_grids), int(pool_bytes), stream, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().tws_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, width: int, height: int, params: Params | None = None) -> int:
+        gid = C.c_int32()
+        p = params if params is not None else params_physical()
+        check(lib().tws_batch_add(self._h, int(width), int(height), C.byref(p), C.byref(gid)))
+        self.shapes[gid.value] = (int(height), int(width))
+        return gid.value
+
+    def try_add(self, width, height, params=None):
+        """Like add(), but returns None when the grid does not fit."""
+        try:
+            return self.add(width, height, params)
+        except TwsError as e:
+            if e.code == TWS_ERR_CAPACITY:
+                return None
+            raise
+
+    def remove(self, gid: int):
+        check(lib().tws_batch_remove(self._h, int(gid)))
+        self.shapes.pop(gid, None)
+
+    def upload(self, gid, terrain=None, depth=None, flux=None):
+        shp = self.shapes[gid]
+        t = _f32(terrain, shp) if terrain is not None else None
+        d = _f32(depth, shp) if depth is not None else None
+        f = _f32(flux, shp + (4,)) if flux is not None else None
+        check(lib().tws_batch_upload(self._h, int(gid), _ptr(t), _ptr(d), _ptr(f)))
+
+    def download(self, gid, depth=True, flux=True, velocity=False):
+        shp = self.shapes[gid]
+        d = np.empty(shp, np.float32) if depth else None
+        f = np.empty(shp + (4,), np.float32) if flux else None
+        v = np.empty(shp + (2,), np.float32) if velocity else None
+        check(lib().tws_batch_download(self._h, int(gid), _ptr(d), _ptr(f), _ptr(v)))
+        return {"depth": d, "flux": f, "velocity": v}
+
+    def set_params(self, gid, p: Params):
+        check(lib().tws_batch_set_params(self._h, int(gid), C.byref(p)))
+
+    def set_rain(self, gid, cx, cy, intensity, radius_sq):
+        check(lib().tws_batch_set_rain(self._h, int(gid), cx, cy, intensity, radius_sq))
+
+    def volume(self, gid) -> float:
+        v = C.c_double()
+        check(lib().tws_batch_volume(self._h, int(gid), C.byref(v)))
+        return v.value
+
+    def check_finite(self, gid) -> bool:
+        code = lib().tws_batch_check_finite(self._h, int(gid))
+        if code == -4:
+            return False
+        check(code)
+        return True
+
+    def set_steps(self, gid, steps: int):
+        check(lib().tws_batch_set_steps(self._h, int(gid), int(steps)))
+
+    def run(self):
+        check(lib().tws_batch_run(self._h))
+
+    def step(self, n: int):
+        check(lib().tws_batch_step(self._h, int(n)))
+
+    def set_launch(self, kernel: int, steps_per_launch: int):
+        check(lib().tws_batch_set_launch(self._h, kernel, steps_per_launch))
+
+    def get_launch(self):
+        k, s = C.c_int32(), C.c_int32()
+        check(lib().tws_batch_get_launch(self._h, C.byref(k), C.byref(s)))
+        return k.value, s.value
+
+    def use_graphs(self, enable: bool):
+        check(lib().tws_batch_use_graphs(self._h, 1 if enable else 0))
+
+    def stats(self) -> dict:
+        st = BatchStats()
+        check(lib().tws_batch_stats_get(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def sync(self):
+        check(lib().tws_batch_sync(self._h))
+
+
+class Server:
+    """Continuous-batching request loop (tws_server_*).  Inputs are copied at
+    submit; output arrays are kept alive here until the request finishes."""
+
+    def __init__(self, pool_bytes: int, max_resident: int = 64, chunk_steps: int = 100,
+                 kernel: int = 0, steps_per_launch: int = 0, max_queue: int = 0):
+        cfg = ServerConfig(int(pool_bytes), int(max_resident), int(chunk_steps), int(kernel),
+                           int(steps_per_launch), int(max_queue))
+        self._h = C.c_void_p()
+        self._keep = {}
+        check(lib().tws_server_create(C.byref(cfg), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().tws_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, terrain, steps, params=None, depth=None, flux=None, rain=None,
+               want_flux=True, want_velocity=False):
+        """Returns (ticket, outputs) where outputs holds the arrays the server
+        fills when the request completes."""
+        H, W = np.shape(terrain)
+        t = _f32(terrain, (H, W))
+        d = _f32(depth, (H, W)) if depth is not None else None
+        f = _f32(flux, (H, W, 4)) if flux is not None else None
+        out = {"depth": np.zeros((H, W), np.float32),
+               "flux": np.zeros((H, W, 4), np.float32) if want_flux else None,
+               "velocity": np.zeros((H, W, 2), np.float32) if want_velocity else None}
+        r = Request()
+        r.width, r.height, r.steps = W, H, int(steps)
+        r.params = params if params is not None else params_physical()
+        r.terrain, r.depth, r.flux4 = _ptr(t), _ptr(d), _ptr(f)
+        if rain is not None:
+            r.rain_on = 1
+            for i in range(4):
+                r.rain[i] = rain[i]
+        r.out_depth, r.out_flux4, r.out_velocity2 = _ptr(out["depth"]), _ptr(out["flux"]), \
+            _ptr(out["velocity"])
+        ticket = C.c_uint64()
+        check(lib().tws_server_submit(self._h, C.byref(r), C.byref(ticket)))
+        self._keep[ticket.value] = out
+        return ticket.value, out
+
+    def poll(self, ticket) -> RequestStatus:
+        st = RequestStatus()
+        check(lib().tws_server_poll(self._h, ticket, C.byref(st)))
+        return st
+
+    def wait(self, ticket, timeout=120.0) -> RequestStatus:
+        st = RequestStatus()
+        check(lib().tws_server_wait(self._h, ticket, float(timeout), C.byref(st)))
+        return st
+
+    def cancel(self, ticket):
+        check(lib().tws_server_cancel(self._h, ticket))
+
+    def drain(self):
+        check(lib().tws_server_drain(self._h))
+
+    def now(self) -> float:
+        return lib().tws_server_now(self._h)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().tws_server_stats_get(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}

@@ -18,6 +18,7 @@ TWS_ERR_NOMEM = -2
 TWS_ERR_DEVICE = -3
 TWS_ERR_NONFINITE = -4
 TWS_ERR_STATE = -5
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
 TWS_ERR_CAPACITY = -6
 
 BOUNDARY_CLOSED = 0
@@ -62,6 +63,56 @@ class DevicePlanes(C.Structure):
     ]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("grids", C.c_int32),
+        ("max_grids", C.c_int32),
+        ("cells", C.c_int64),
+        ("pool_bytes", C.c_int64),
+        ("pool_used", C.c_int64),
+        ("pool_largest_free", C.c_int64),
+        ("graph_captures", C.c_int64),
+        ("graph_replays", C.c_int64),
+    ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("pool_bytes", C.c_int64),
+        ("max_resident", C.c_int32),
+        ("chunk_steps", C.c_int32),
+        ("kernel", C.c_int32),
+        ("steps_per_launch", C.c_int32),
+        ("max_queue", C.c_int32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("steps", C.c_int32),
+        ("params", Params),
+        ("terrain", C.c_void_p), ("depth", C.c_void_p), ("flux4", C.c_void_p),
+        ("rain_on", C.c_int32), ("rain", C.c_float * 4),
+        ("out_depth", C.c_void_p), ("out_flux4", C.c_void_p), ("out_velocity2", C.c_void_p),
+    ]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32), ("error", C.c_int32), ("steps_done", C.c_int32), ("pad_", C.c_int32),
+        ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+        ("cancelled", C.c_int64), ("queued", C.c_int64), ("resident", C.c_int64),
+        ("runs", C.c_int64), ("cell_updates", C.c_int64), ("busy_seconds", C.c_double),
+        ("pool_used", C.c_int64), ("pool_bytes", C.c_int64),
+    ]
+
+
 _fp = C.POINTER(C.c_float)
 _vp = C.c_void_p
 _i32 = C.c_int32
@@ -93,6 +144,34 @@ PROTOTYPES = {
     "tws_debug_lane_shift_probe": (C.c_int, [C.POINTER(_i32)]),
     "tws_float_to_half_rtz": (C.c_int, [_vp, _vp, C.c_int64]),
     "tws_stream_copy_bench": (C.c_int, [C.c_int64, _i32, C.POINTER(C.c_double)]),
+    "tws_grid_footprint": (C.c_int64, [_i32, _i32]),
+    "tws_batch_create": (C.c_int, [_i32, C.c_int64, _vp, C.POINTER(_vp)]),
+    "tws_batch_destroy": (C.c_int, [_vp]),
+    "tws_batch_add": (C.c_int, [_vp, _i32, _i32, C.POINTER(Params), C.POINTER(_i32)]),
+    "tws_batch_remove": (C.c_int, [_vp, _i32]),
+    "tws_batch_upload": (C.c_int, [_vp, _i32, _vp, _vp, _vp]),
+    "tws_batch_download": (C.c_int, [_vp, _i32, _vp, _vp, _vp]),
+    "tws_batch_set_params": (C.c_int, [_vp, _i32, C.POINTER(Params)]),
+    "tws_batch_set_rain": (C.c_int, [_vp, _i32, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "tws_batch_volume": (C.c_int, [_vp, _i32, C.POINTER(C.c_double)]),
+    "tws_batch_check_finite": (C.c_int, [_vp, _i32]),
+    "tws_batch_set_steps": (C.c_int, [_vp, _i32, _i32]),
+    "tws_batch_run": (C.c_int, [_vp]),
+    "tws_batch_step": (C.c_int, [_vp, _i32]),
+    "tws_batch_set_launch": (C.c_int, [_vp, _i32, _i32]),
+    "tws_batch_get_launch": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32)]),
+    "tws_batch_use_graphs": (C.c_int, [_vp, _i32]),
+    "tws_batch_stats_get": (C.c_int, [_vp, C.POINTER(BatchStats)]),
+    "tws_batch_sync": (C.c_int, [_vp]),
+    "tws_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "tws_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "tws_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(RequestStatus)]),
+    "tws_server_wait": (C.c_int, [_vp, C.c_uint64, C.c_double, C.POINTER(RequestStatus)]),
+    "tws_server_cancel": (C.c_int, [_vp, C.c_uint64]),
+    "tws_server_drain": (C.c_int, [_vp]),
+    "tws_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
+    "tws_server_now": (C.c_double, [_vp]),
+    "tws_server_destroy": (C.c_int, [_vp]),
     "tws_last_error": (C.c_char_p, []),
     "tws_abi_version": (C.c_int, []),
     "tws_device_count": (C.c_int, []),

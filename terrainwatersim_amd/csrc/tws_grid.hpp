@@ -1,0 +1,59 @@
+// tws_grid.hpp -- device state of one grid and the operations on it, shared
+// by the single-grid handle (tws_sim) and the slots of a batch (tws_batch).
+//
+// Layout: 11 planes of height x pitch floats (pitch = width rounded up to 64,
+// i.e. 256-byte rows) in one block of device memory:
+//   b | d0 | d1 | f0[+x,-x,+y,-y] | f1[+x,-x,+y,-y]
+// d and f are ping-pong sets; `parity` names the set holding the current
+// state.  The idle set doubles as staging for the interleaved host layouts,
+// so no operation here allocates device memory.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "tws.h"
+#include "tws_internal.hpp"
+
+namespace tws {
+
+constexpr int kPlanes = 11;
+
+struct GridState {
+    int W = 0, H = 0;
+    int64_t pitch = 0;
+    tws_params params{};
+    float* b = nullptr;
+    float* d[2] = {nullptr, nullptr};
+    float* f[2][4] = {};
+    unsigned* status = nullptr;  // device flag word (bit 0: non-finite seen)
+    int parity = 0;
+    int rain_on = 0;
+    float rain[4] = {0, 0, 0, 0};
+
+    static size_t bytes_for(int W, int H) {
+        return sizeof(float) * (size_t)kPlanes * (size_t)pitch_of(W) * (size_t)H;
+    }
+    static int64_t pitch_of(int W) { return ((int64_t)W + 63) / 64 * 64; }
+    void bind(float* base, int w, int h);   // plane pointers inside `base`
+    GridDesc desc() const;                  // kernel view, parity0 = parity
+    float* staging() const { return f[parity ^ 1][0]; }  // >= 4*W*H floats
+};
+
+int grid_clear(GridState& g, hipStream_t s);
+int grid_upload(GridState& g, const float* terrain, const float* depth, const float* flux4,
+                hipStream_t s);
+int grid_download(GridState& g, float* depth, float* flux4, float* velocity2, hipStream_t s);
+int grid_upload_reference(GridState& g, const float* terrain_rgba, const float* flow_rgba,
+                          hipStream_t s);
+int grid_download_reference(GridState& g, float* terrain_rgba, float gb_fill, float* flow_rgba,
+                            uint16_t* flowmap, hipStream_t s);
+int grid_brush(GridState& g, float cx, float cy, float I, float R2, hipStream_t s);
+int grid_set_rain(GridState& g, float cx, float cy, float I, float R2);
+int grid_volume(GridState& g, double* partials_dev, double* out, hipStream_t s);
+int grid_check_finite(GridState& g, hipStream_t s);  // reads and clears the flag
+
+// Tiles of a grid for a (kernel variant, steps per launch) pair.
+bool grid_tiles(int W, int H, int kernel, int k, int* tiles_x, int* tiles_y);
+
+}  // namespace tws

@@ -74,8 +74,10 @@ struct TileShape {
 bool tile_shape(int kernel, int k, TileShape* out);
 
 // Launchers (stream-ordered, return hipError_t).
-hipError_t launch_tile(int kernel, int k, const GridDesc* grids_dev, int ngrids,
-                       int total_tiles, int launch_idx, hipStream_t s);
+// Batch launch: hdr_dev = {number of grids, number of tiles}; `blocks` is an
+// upper bound of the tile count (extra blocks exit).
+hipError_t launch_tile_batch(int kernel, int k, const GridDesc* grids_dev, const int* hdr_dev,
+                             int blocks, int launch_idx, hipStream_t s);
 hipError_t launch_twopass(const GridDesc* g_dev, const GridDesc& g_host, int parity,
                           hipStream_t s);
 hipError_t launch_brush(const GridDesc& g, int parity, float cx, float cy, float I, float R2,

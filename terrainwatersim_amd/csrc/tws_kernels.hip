@@ -124,11 +124,14 @@ __device__ __forceinline__ void flux_update(float F, float K, float C, float d, 
 
 // ---------------------------------------------------------------------------
 // Fused, temporally blocked step.  Grid: one block per tile of the flattened
-// tile list (all grids of a batch).  g1 is used when ngrids == 0 (a single
-// grid passed by value); otherwise grids[] is searched by tile_begin.
+// tile list.  Single grid: `grids` is null, g1 (by value) is the grid and
+// the tile count is gridDim.x.  Batch: hdr = {number of grids, number of
+// tiles} and grids[] (ordered by tile_begin) live in device memory, so a
+// captured launch stays valid while grids come and go; blocks past the tile
+// count exit at once.
 template <int CPL, int RPW, int NW, int KH>
 __global__ void __launch_bounds__(NW * kWave)
-k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_tiles,
+k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ hdr,
        int launch_idx) {
     using G = TileGeom<CPL, RPW, NW, KH>;
     constexpr int EXW = G::EXW, EXH = G::EXH;
@@ -139,13 +142,16 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, int ngrids, int total_ti
     // XCD-aware order: the hardware deals blocks to the 8 XCDs round robin;
     // give each XCD a contiguous run of tiles so neighbouring tiles (whose
     // halos overlap) share one L2.
+    const int ngrids = grids ? hdr[0] : 0;
+    const int total_tiles = grids ? hdr[1] : (int)gridDim.x;
     const int bid = blockIdx.x;
+    if (bid >= total_tiles) return;
     const int xcd = bid & 7, idx = bid >> 3;
     const int q = total_tiles >> 3, rmd = total_tiles & 7;
     const int tile = xcd * q + (xcd < rmd ? xcd : rmd) + idx;
 
     GridDesc g;
-    if (ngrids == 0) {
+    if (!grids) {
         g = g1;
     } else {
         int lo = 0, hi = ngrids - 1;
@@ -567,19 +573,19 @@ __global__ void __launch_bounds__(64) k_lane_shift_probe(int* out) {
 static dim3 cell_grid(const GridDesc& g) { return dim3((g.W + 15) / 16, (g.H + 15) / 16); }
 
 template <int CPL, int RPW, int NW>
-static hipError_t launch_tile_k(int k, const GridDesc* grids_dev, int ngrids,
-                                const GridDesc* single, int total_tiles, int launch_idx,
+static hipError_t launch_tile_k(int k, const GridDesc* grids_dev, const int* hdr,
+                                const GridDesc* single, int blocks, int launch_idx,
                                 hipStream_t s) {
     GridDesc g1{};
     if (single) g1 = *single;
-    const dim3 grid(total_tiles), block(NW * kWave);
+    const dim3 grid(blocks), block(NW * kWave);
     switch (k) {
-        case 1: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 1>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
-        case 2: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 2>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
-        case 4: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 4>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
-        case 3: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 3>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
-        case 6: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 6>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
-        case 8: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 8>), grid, block, 0, s, g1, grids_dev, ngrids, total_tiles, launch_idx); break;
+        case 1: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 1>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 2: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 2>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 4: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 4>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 3: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 3>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 6: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 6>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 8: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 8>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -609,26 +615,26 @@ bool tile_shape(int kernel, int k, TileShape* out) {
     }
 }
 
-static hipError_t launch_tile_any(int kernel, int k, const GridDesc* grids_dev, int ngrids,
-                                  const GridDesc* single, int total_tiles, int launch_idx,
+static hipError_t launch_tile_any(int kernel, int k, const GridDesc* grids_dev, const int* hdr,
+                                  const GridDesc* single, int blocks, int launch_idx,
                                   hipStream_t s) {
-    if (total_tiles <= 0) return hipSuccess;
+    if (blocks <= 0) return hipSuccess;
     switch (kernel) {
-        case TWS_KERNEL_TILE_128x128: return launch_tile_k<2, 8, 16>(k, grids_dev, ngrids, single, total_tiles, launch_idx, s);
-        case TWS_KERNEL_TILE_128x64: return launch_tile_k<2, 8, 8>(k, grids_dev, ngrids, single, total_tiles, launch_idx, s);
-        case TWS_KERNEL_TILE_256x64: return launch_tile_k<4, 4, 16>(k, grids_dev, ngrids, single, total_tiles, launch_idx, s);
+        case TWS_KERNEL_TILE_128x128: return launch_tile_k<2, 8, 16>(k, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_128x64: return launch_tile_k<2, 8, 8>(k, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_256x64: return launch_tile_k<4, 4, 16>(k, grids_dev, hdr, single, blocks, launch_idx, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_tile(int kernel, int k, const GridDesc* grids_dev, int ngrids, int total_tiles,
-                       int launch_idx, hipStream_t s) {
-    return launch_tile_any(kernel, k, grids_dev, ngrids, nullptr, total_tiles, launch_idx, s);
+hipError_t launch_tile_batch(int kernel, int k, const GridDesc* grids_dev, const int* hdr_dev,
+                             int blocks, int launch_idx, hipStream_t s) {
+    return launch_tile_any(kernel, k, grids_dev, hdr_dev, nullptr, blocks, launch_idx, s);
 }
 
 hipError_t launch_tile_single(int kernel, int k, const GridDesc& g, int launch_idx,
                               hipStream_t s) {
-    return launch_tile_any(kernel, k, nullptr, 0, &g, g.tiles_x * g.tiles_y, launch_idx, s);
+    return launch_tile_any(kernel, k, nullptr, nullptr, &g, g.tiles_x * g.tiles_y, launch_idx, s);
 }
 
 hipError_t launch_twopass(const GridDesc* /*g_dev*/, const GridDesc& g, int parity,

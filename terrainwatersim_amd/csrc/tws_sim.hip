@@ -1,0 +1,275 @@
+// tws_sim.hip -- the single-grid handle of the C ABI (tws.h): one grid's
+// device state (tws_grid.hpp), its stream and the step planner.
+
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <vector>
+
+#include "tws.h"
+#include "tws_grid.hpp"
+#include "tws_runtime.hpp"
+
+using namespace tws;
+
+struct tws_sim {
+    GridState g;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    float* base = nullptr;       // the grid's planes
+    double* aux = nullptr;       // volume partials + status word
+    int kernel = TWS_KERNEL_TILE_128x64;
+    int k = 4;
+    double time_acc = 0.0;       // PerformSimulationStep accumulator
+};
+
+#define TWS_HIP(call, what)                              \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+namespace tws {
+
+// Runs n steps of one grid as a sequence of launches: full launches of k
+// steps, then the tail on the smallest template that holds it.  With `ev`
+// (2 events per launch) each launch is bracketed by an event pair.
+int step_grid(GridState& gs, int kernel, int k, int32_t n, hipStream_t stream, hipEvent_t* ev,
+              int ev_pairs, int* launches) {
+    if (n < 0) return fail(TWS_ERR_INVALID, "step: negative step count %d", n);
+    int nl = 0;
+    auto rec = [&](int which) -> hipError_t {
+        if (!ev || nl >= ev_pairs) return hipSuccess;
+        return hipEventRecord(ev[2 * nl + which], stream);
+    };
+    if (kernel == TWS_KERNEL_TWOPASS) {
+        for (int i = 0; i < n; ++i) {
+            const GridDesc g = gs.desc();
+            TWS_HIP(rec(0), "step: event");
+            TWS_HIP(launch_twopass(nullptr, g, gs.parity, stream), "step: two-pass kernels");
+            TWS_HIP(rec(1), "step: event");
+            gs.parity ^= 1;
+            ++nl;
+        }
+        if (launches) *launches = nl;
+        return TWS_OK;
+    }
+    int left = n;
+    while (left > 0) {
+        int kk = k;
+        if (left < kk) {
+            static const int ladder[] = {1, 2, 3, 4, 6, 8};
+            for (int c : ladder)
+                if (c >= left) { kk = c; break; }
+        }
+        const int levels = left < kk ? left : kk;
+        GridDesc g = gs.desc();
+        if (!grid_tiles(gs.W, gs.H, kernel, kk, &g.tiles_x, &g.tiles_y))
+            return fail(TWS_ERR_INVALID, "step: bad launch config %d/%d", kernel, kk);
+        g.tile_begin = 0;
+        g.steps = levels;
+        TWS_HIP(rec(0), "step: event");
+        TWS_HIP(launch_tile_single(kernel, kk, g, 0, stream), "step: tile kernel");
+        TWS_HIP(rec(1), "step: event");
+        gs.parity ^= 1;
+        left -= levels;
+        ++nl;
+    }
+    if (launches) *launches = nl;
+    return TWS_OK;
+}
+
+}  // namespace tws
+
+extern "C" {
+
+int tws_create(int32_t width, int32_t height, const tws_params* params, void* stream,
+               tws_sim** out) {
+    if (!out) return fail(TWS_ERR_INVALID, "tws_create: out is NULL");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || width > (1 << 20) || height > (1 << 20))
+        return fail(TWS_ERR_INVALID, "tws_create: invalid size %d x %d", width, height);
+    if (!valid_params(params)) return fail(TWS_ERR_INVALID, "tws_create: invalid params");
+    tws_sim* s = new (std::nothrow) tws_sim();
+    if (!s) return fail(TWS_ERR_NOMEM, "tws_create: host allocation failed");
+    hipError_t e = hipSuccess;
+    if (stream) {
+        s->stream = (hipStream_t)stream;
+    } else {
+        e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { delete s; return hip_fail(e, "tws_create: hipStreamCreate"); }
+        s->own_stream = true;
+    }
+    e = hipMalloc(&s->base, GridState::bytes_for(width, height));
+    if (e != hipSuccess) { tws_destroy(s); return hip_fail(e, "tws_create: hipMalloc state"); }
+    e = hipMalloc(&s->aux, sizeof(double) * kVolumeParts + 256);
+    if (e != hipSuccess) { tws_destroy(s); return hip_fail(e, "tws_create: hipMalloc aux"); }
+    s->g.bind(s->base, width, height);
+    s->g.params = *params;
+    s->g.status = reinterpret_cast<unsigned*>(s->aux + kVolumeParts);
+    int rc = grid_clear(s->g, s->stream);
+    if (rc == TWS_OK) {
+        e = hipMemsetAsync(s->g.status, 0, 256, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "tws_create: clear state");
+    }
+    if (rc != TWS_OK) { tws_destroy(s); return rc; }
+    choose_launch(width, height, &s->kernel, &s->k);
+    *out = s;
+    return TWS_OK;
+}
+
+int tws_destroy(tws_sim* s) {
+    if (!s) return TWS_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->base) (void)hipFree(s->base);
+    if (s->aux) (void)hipFree(s->aux);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return TWS_OK;
+}
+
+int tws_set_params(tws_sim* s, const tws_params* p) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_set_params: NULL handle");
+    if (!valid_params(p)) return fail(TWS_ERR_INVALID, "tws_set_params: invalid params");
+    s->g.params = *p;
+    return TWS_OK;
+}
+
+int tws_get_params(const tws_sim* s, tws_params* out) {
+    if (!s || !out) return fail(TWS_ERR_INVALID, "tws_get_params: NULL argument");
+    *out = s->g.params;
+    return TWS_OK;
+}
+
+int tws_upload(tws_sim* s, const float* terrain, const float* depth, const float* flux4) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_upload: NULL handle");
+    return grid_upload(s->g, terrain, depth, flux4, s->stream);
+}
+
+int tws_download(tws_sim* s, float* depth, float* flux4, float* velocity2) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_download: NULL handle");
+    return grid_download(s->g, depth, flux4, velocity2, s->stream);
+}
+
+int tws_upload_reference_layout(tws_sim* s, const float* terrain_rgba32f,
+                                const float* flow_rgba32f) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_upload_reference_layout: NULL handle");
+    return grid_upload_reference(s->g, terrain_rgba32f, flow_rgba32f, s->stream);
+}
+
+int tws_download_reference_layout(tws_sim* s, float* terrain_rgba32f, float gb_fill,
+                                  float* flow_rgba32f, uint16_t* flowmap_rg16f) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_download_reference_layout: NULL handle");
+    return grid_download_reference(s->g, terrain_rgba32f, gb_fill, flow_rgba32f, flowmap_rg16f,
+                                   s->stream);
+}
+
+int tws_apply_brush(tws_sim* s, float cx, float cy, float intensity, float radius_sq) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_apply_brush: NULL handle");
+    return grid_brush(s->g, cx, cy, intensity, radius_sq, s->stream);
+}
+
+int tws_set_rain(tws_sim* s, float cx, float cy, float intensity, float radius_sq) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_set_rain: NULL handle");
+    return grid_set_rain(s->g, cx, cy, intensity, radius_sq);
+}
+
+int tws_step(tws_sim* s, int32_t n) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_step: NULL handle");
+    return step_grid(s->g, s->kernel, s->k, n, s->stream, nullptr, 0, nullptr);
+}
+
+int tws_step_profiled(tws_sim* s, int32_t n, float* launch_ms, int32_t capacity,
+                      int32_t* launches) {
+    if (!s || !launch_ms || capacity <= 0 || !launches)
+        return fail(TWS_ERR_INVALID, "tws_step_profiled: invalid argument");
+    std::vector<hipEvent_t> ev(2 * (size_t)capacity, nullptr);
+    for (auto& e : ev) {
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) {
+            for (auto& x : ev) if (x) (void)hipEventDestroy(x);
+            return hip_fail(r, "tws_step_profiled: hipEventCreate");
+        }
+    }
+    int nl = 0;
+    int rc = step_grid(s->g, s->kernel, s->k, n, s->stream, ev.data(), capacity, &nl);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    const int m = nl < capacity ? nl : capacity;
+    for (int i = 0; rc == TWS_OK && e == hipSuccess && i < m; ++i)
+        e = hipEventElapsedTime(&launch_ms[i], ev[2 * i], ev[2 * i + 1]);
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    if (rc != TWS_OK) return rc;
+    if (e != hipSuccess) return hip_fail(e, "tws_step_profiled: timing");
+    *launches = nl;
+    return TWS_OK;
+}
+
+int tws_perform_simulation_step(tws_sim* s, double frame_seconds, double step_seconds,
+                                int32_t* steps_done) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_perform_simulation_step: NULL handle");
+    if (!(step_seconds > 0.0) || !(frame_seconds >= 0.0))
+        return fail(TWS_ERR_INVALID, "tws_perform_simulation_step: invalid times");
+    // Terrain.cpp:242-247: accumulate, take whole steps, then clamp to 10
+    // (time beyond the clamp is dropped, as in the reference).
+    s->time_acc += frame_seconds;
+    uint32_t n = (uint32_t)(s->time_acc / step_seconds);
+    s->time_acc -= step_seconds * n;
+    if (n > 10) n = 10;
+    if (steps_done) *steps_done = (int32_t)n;
+    return tws_step(s, (int32_t)n);
+}
+
+int tws_volume(tws_sim* s, double* out) {
+    if (!s || !out) return fail(TWS_ERR_INVALID, "tws_volume: NULL argument");
+    return grid_volume(s->g, s->aux, out, s->stream);
+}
+
+int tws_check_finite(tws_sim* s) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_check_finite: NULL handle");
+    return grid_check_finite(s->g, s->stream);
+}
+
+int tws_sync(tws_sim* s) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_sync: NULL handle");
+    TWS_HIP(hipStreamSynchronize(s->stream), "tws_sync");
+    return TWS_OK;
+}
+
+int tws_get_device_planes(const tws_sim* s, tws_device_planes* out) {
+    if (!s || !out) return fail(TWS_ERR_INVALID, "tws_get_device_planes: NULL argument");
+    out->terrain = s->g.b;
+    out->depth = s->g.d[s->g.parity];
+    for (int c = 0; c < 4; ++c) out->flux[c] = s->g.f[s->g.parity][c];
+    out->pitch = s->g.pitch;
+    out->width = s->g.W;
+    out->height = s->g.H;
+    return TWS_OK;
+}
+
+int tws_set_launch(tws_sim* s, int32_t kernel, int32_t k) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_set_launch: NULL handle");
+    if (kernel == TWS_KERNEL_AUTO) {
+        choose_launch(s->g.W, s->g.H, &s->kernel, &s->k);
+        return TWS_OK;
+    }
+    if (kernel == TWS_KERNEL_TWOPASS) {
+        if (k != 1) return fail(TWS_ERR_INVALID, "tws_set_launch: TWOPASS runs 1 step per launch");
+    } else {
+        TileShape sh{};
+        if (!tile_shape(kernel, k, &sh))
+            return fail(TWS_ERR_INVALID, "tws_set_launch: unknown kernel %d or k %d", kernel, k);
+    }
+    s->kernel = kernel;
+    s->k = k;
+    return TWS_OK;
+}
+
+int tws_get_launch(const tws_sim* s, int32_t* kernel, int32_t* k) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_get_launch: NULL handle");
+    if (kernel) *kernel = s->kernel;
+    if (k) *k = s->k;
+    return TWS_OK;
+}
+
+}  // extern "C"

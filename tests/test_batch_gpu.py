@@ -1,0 +1,145 @@
+"""Batched form: many grids in one launch sequence, bit-identical to each grid
+stepped alone (and so to the oracle and the reference shaders)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+GB = 1 << 30
+
+
+def state(seed, H, W):
+    rng = np.random.default_rng(seed)
+    b = (rng.random((H, W)) * 5.0).astype(F32)
+    d = (rng.random((H, W)) * 2.0 * (rng.random((H, W)) < 0.6)).astype(F32)
+    f = (rng.random((H, W, 4)) * 0.3).astype(F32)
+    return b, d, f
+
+
+SHAPES = [(1, 1), (37, 53), (256, 256), (129, 300), (700, 1024), (64, 2048)]
+
+
+def make_grids(tws):
+    grids = []
+    for i, (H, W) in enumerate(SHAPES):
+        b, d, f = state(100 + i, H, W)
+        boundary = tws.BOUNDARY_ZERO if i % 3 == 1 else tws.BOUNDARY_CLOSED
+        p = tws.params_physical(0.02 + 0.005 * i, 9.81, 1.0, 1.0, 0.995, boundary)
+        rain = (W / 2.0, H / 3.0, 1e-2, 64.0) if i % 2 == 0 else None
+        grids.append((b, d, f, p, rain))
+    return grids
+
+
+def alone(tws, b, d, f, p, rain, n):
+    H, W = b.shape
+    with tws.Simulation(W, H, p) as s:
+        s.upload(b, d, f)
+        if rain:
+            s.set_rain(*rain)
+        s.step(n)
+        return s.download(flux=True)
+
+
+def load_batch(tws, batch, grids):
+    ids = []
+    for b, d, f, p, rain in grids:
+        H, W = b.shape
+        gid = batch.add(W, H, p)
+        batch.upload(gid, b, d, f)
+        if rain:
+            batch.set_rain(gid, *rain)
+        ids.append(gid)
+    return ids
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_equals_alone(gpu, graphs):
+    grids = make_grids(gpu)
+    with gpu.Batch(16, 1 * GB) as batch:
+        batch.use_graphs(graphs)
+        ids = load_batch(gpu, batch, grids)
+        batch.step(13)
+        for gid, (b, d, f, p, rain) in zip(ids, grids):
+            out = batch.download(gid)
+            ref = alone(gpu, b, d, f, p, rain, 13)
+            assert np.array_equal(out["depth"], ref["depth"]), b.shape
+            assert np.array_equal(out["flux"], ref["flux"]), b.shape
+            assert batch.check_finite(gid)
+
+
+def test_per_grid_step_counts(gpu):
+    grids = make_grids(gpu)
+    counts = [0, 1, 5, 17, 8, 3]
+    with gpu.Batch(16, 1 * GB) as batch:
+        ids = load_batch(gpu, batch, grids)
+        for gid, n in zip(ids, counts):
+            batch.set_steps(gid, n)
+        batch.run()
+        for gid, n, (b, d, f, p, rain) in zip(ids, counts, grids):
+            ref = alone(gpu, b, d, f, p, rain, n)
+            out = batch.download(gid)
+            assert np.array_equal(out["depth"], ref["depth"]) and np.array_equal(out["flux"], ref["flux"])
+
+
+def test_graph_replays_match_eager(gpu):
+    """The captured sequence replayed several times, with eager work and
+    synchronisation between replays, equals eager launches every time."""
+    grids = make_grids(gpu)[:4]
+    with gpu.Batch(8, GB) as g_batch, gpu.Batch(8, GB) as e_batch:
+        e_batch.use_graphs(False)
+        gi = load_batch(gpu, g_batch, grids)
+        ei = load_batch(gpu, e_batch, grids)
+        for rep in range(4):
+            g_batch.step(8)
+            e_batch.step(8)
+            g_batch.sync()
+            for a, c in zip(gi, ei):
+                assert np.array_equal(g_batch.download(a)["depth"], e_batch.download(c)["depth"]), rep
+        st = g_batch.stats()
+        assert st["graph_captures"] == 1 and st["graph_replays"] == 4
+
+
+def test_chunks_equal_one_run(gpu):
+    grids = make_grids(gpu)[:3]
+    with gpu.Batch(8, GB) as batch:
+        ids = load_batch(gpu, batch, grids)
+        for _ in range(5):
+            batch.step(7)
+        for gid, (b, d, f, p, rain) in zip(ids, grids):
+            assert np.array_equal(batch.download(gid)["depth"], alone(gpu, b, d, f, p, rain, 35)["depth"])
+
+
+def test_capacity_and_reuse(gpu):
+    fp = gpu.grid_footprint(512, 512)
+    with gpu.Batch(3, 2 * fp) as batch:
+        a = batch.add(512, 512)
+        b_ = batch.add(512, 512)
+        assert batch.try_add(512, 512) is None            # pool full
+        assert batch.try_add(8, 8) is None                 # pool full (no room at all)
+        batch.remove(a)
+        c = batch.add(256, 256)                            # fits in the freed block
+        z = batch.download(c)
+        assert not z["depth"].any() and not z["flux"].any()  # a fresh grid starts at zero
+        st = batch.stats()
+        assert st["grids"] == 2 and st["pool_used"] <= st["pool_bytes"]
+        batch.remove(b_)
+        batch.remove(c)
+        assert batch.stats()["pool_used"] == 0
+        d = batch.add(700, 300)                            # coalesced again
+        assert d >= 0
+
+
+def test_nan_in_one_grid_is_isolated(gpu):
+    grids = make_grids(gpu)[:3]
+    bad_b, bad_d, bad_f, p, rain = grids[1]
+    bad_d = bad_d.copy()
+    bad_d[3, 4] = np.inf
+    grids[1] = (bad_b, bad_d, bad_f, p, rain)
+    with gpu.Batch(8, GB) as batch:
+        ids = load_batch(gpu, batch, grids)
+        batch.step(5)
+        assert batch.check_finite(ids[0]) and batch.check_finite(ids[2])
+        assert not batch.check_finite(ids[1])
+        b, d, f, p, rain = grids[0]
+        assert np.array_equal(batch.download(ids[0])["depth"], alone(gpu, b, d, f, p, rain, 5)["depth"])

@@ -1,0 +1,108 @@
+"""Serving form: continuous batching over a device pool.  Each result equals
+the same solve run alone (bitwise); arrivals join a running batch; failure
+handling is per request."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+GB = 1 << 30
+
+
+def request(tws, seed, H, W):
+    rng = np.random.default_rng(seed)
+    b = tws.generate_fbm_terrain(W, H, seed=seed, height_scale=20.0)
+    d = (rng.random((H, W)) * 0.5).astype(F32)
+    return b, d
+
+
+def alone(tws, b, d, steps, p, rain=None):
+    H, W = b.shape
+    with tws.Simulation(W, H, p) as s:
+        s.upload(b, d)
+        if rain:
+            s.set_rain(*rain)
+        s.step(steps)
+        return s.download(flux=True)
+
+
+def test_results_equal_alone(gpu):
+    p = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995)
+    jobs = [(1, 96, 160, 37, None), (2, 256, 256, 250, (128.0, 128.0, 1e-3, 256.0)),
+            (3, 33, 47, 5, None), (4, 512, 384, 120, None), (5, 200, 100, 0, None)]
+    with gpu.Server(2 * GB, max_resident=8, chunk_steps=16) as srv:
+        sub = []
+        for seed, H, W, n, rain in jobs:
+            b, d = request(gpu, seed, H, W)
+            t, out = srv.submit(b, n, p, depth=d, rain=rain)
+            sub.append((t, out, b, d, n, rain))
+        srv.drain()
+        for t, out, b, d, n, rain in sub:
+            st = srv.poll(t)
+            assert st.state == gpu.REQ_DONE and st.steps_done == n
+            assert st.t_submit <= st.t_admit <= st.t_done
+            ref = alone(gpu, b, d, n, p, rain)
+            assert np.array_equal(out["depth"], ref["depth"]) and np.array_equal(out["flux"], ref["flux"])
+        stats = srv.stats()
+        assert stats["completed"] == len(jobs) and stats["failed"] == 0
+        assert stats["cell_updates"] == sum(H * W * n for _, H, W, n, _ in jobs)
+
+
+def test_arrivals_join_running_batch(gpu):
+    """A long solve is resident; short solves submitted while it runs are
+    admitted and finish before it (continuous batching, not a FIFO queue)."""
+    p = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995)
+    with gpu.Server(2 * GB, max_resident=8, chunk_steps=20) as srv:
+        b, d = request(gpu, 10, 1024, 1024)
+        t_long, out_long = srv.submit(b, 3000, p, depth=d)
+        while srv.poll(t_long).state == gpu.REQ_QUEUED:
+            pass
+        shorts = []
+        for i in range(3):
+            bs, ds = request(gpu, 20 + i, 128, 192)
+            shorts.append(srv.submit(bs, 40, p, depth=ds) + (bs, ds))
+        for t, out, bs, ds in shorts:
+            st = srv.wait(t)
+            assert st.state == gpu.REQ_DONE
+            assert np.array_equal(out["depth"], alone(gpu, bs, ds, 40, p)["depth"])
+        st_long = srv.wait(t_long, timeout=300)
+        assert st_long.state == gpu.REQ_DONE
+        assert all(srv.poll(t).t_done < st_long.t_done for t, *_ in shorts)
+        assert all(srv.poll(t).t_admit > srv.poll(t_long).t_admit for t, *_ in shorts)
+        assert np.array_equal(out_long["depth"], alone(gpu, b, d, 3000, p)["depth"])
+
+
+def test_admission_by_footprint(gpu):
+    p = gpu.params_physical()
+    fp = gpu.grid_footprint(512, 512)
+    with gpu.Server(2 * fp, max_resident=8, chunk_steps=10) as srv:
+        b, d = request(gpu, 30, 512, 512)
+        with pytest.raises(gpu.TwsError):          # can never fit
+            srv.submit(np.zeros((1024, 1024), F32), 5, p)
+        tickets = [srv.submit(b, 30, p, depth=d)[0] for _ in range(4)]
+        srv.drain()
+        sts = [srv.poll(t) for t in tickets]
+        assert all(s.state == gpu.REQ_DONE for s in sts)
+        # at most two fit at once: the third was admitted after one finished
+        admits = sorted(s.t_admit for s in sts)
+        dones = sorted(s.t_done for s in sts)
+        assert admits[2] >= dones[0]
+
+
+def test_cancel_and_nonfinite_are_isolated(gpu):
+    p = gpu.params_physical()
+    with gpu.Server(GB, max_resident=8, chunk_steps=10) as srv:
+        b, d = request(gpu, 40, 256, 256)
+        bad = d.copy()
+        bad[5, 5] = np.nan
+        t_ok, out_ok = srv.submit(b, 60, p, depth=d)
+        t_bad, _ = srv.submit(b, 60, p, depth=bad)
+        t_cancel, _ = srv.submit(b, 100000, p, depth=d)
+        srv.cancel(t_cancel)
+        srv.drain()
+        assert srv.poll(t_ok).state == gpu.REQ_DONE
+        st_bad = srv.poll(t_bad)
+        assert st_bad.state == gpu.REQ_FAILED and st_bad.error == -4
+        assert srv.poll(t_cancel).state == gpu.REQ_CANCELLED
+        assert np.array_equal(out_ok["depth"], alone(gpu, b, d, 60, p)["depth"])

@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB STREAM copy kernel (reference for PMC byte counts)")
+    ap.add_argument("--extra", default="batched,served",
+                    help="comma list of extra workloads in the same line: batched, served, "
+                         "large ('' for none)")
+    ap.add_argument("--served-requests", type=int, default=200)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -169,6 +173,17 @@ def main():
         rec["cpu_baseline"] = cpu_baseline(b, d, (p.flow_friction, p.water_acceleration,
                                                   p.cell_area_inv_timescaled),
                                            target_s=args.cpu_seconds)
+    del b, d
+    extra = [w for w in args.extra.split(",") if w]
+    if extra:
+        from terrainwatersim_amd import workloads
+        if "batched" in extra:
+            rec["batched"] = workloads.batched(log=log)
+        if "served" in extra:
+            cap = rec.get("batched", {}).get("batched_cell_updates_per_s", value)
+            rec["served"] = workloads.served(cap, n_requests=args.served_requests, log=log)
+        if "large" in extra:
+            rec["large"] = workloads.large(log=log)
     print(json.dumps(rec), flush=True)
 
 

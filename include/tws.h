@@ -254,6 +254,28 @@ int tws_batch_use_graphs(tws_batch* batch, int32_t enable);
 int tws_batch_stats_get(const tws_batch* batch, tws_batch_stats* out);
 int tws_batch_sync(tws_batch* batch);
 
+/* ---- autotuning ------------------------------------------------------------ */
+
+typedef struct tws_tune_entry {
+    int32_t kernel;             /* TWS_KERNEL_*                                */
+    int32_t steps_per_launch;
+    double cell_updates_per_s;
+} tws_tune_entry;
+
+/* Times every (tile variant, steps per launch) candidate on scratch grids of
+ * the n given shapes advanced together as one batch (synthetic state: seeded
+ * fBm terrain under a 0.5 m water sheet), `steps` steps per trial, and
+ * returns the fastest in *best.  results[0..capacity) (may be NULL) receives
+ * every trial; *count (may be NULL) their number.  The winner is recorded for
+ * the size class (log2 of cells) of the average grid, and TWS_KERNEL_AUTO
+ * picks it from then on in this process.  Allocates and frees its own
+ * device memory; never touches existing simulations. */
+int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32_t steps,
+                 tws_tune_entry* best, tws_tune_entry* results, int32_t capacity,
+                 int32_t* count);
+/* Drops every recorded autotuning result. */
+int tws_tune_forget(void);
+
 /* ---- serving: a request loop over a batch --------------------------------- */
 /*
  * No counterpart in the reference.  A server owns a batch and a thread.

@@ -15,7 +15,7 @@ from ._lib import (BOUNDARY_CLOSED, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x
                    KERNEL_TILE_128x128, KERNEL_TILE_256x64, KERNEL_TWOPASS, LIB_PATH,
                    PROTOTYPES, REQ_CANCELLED, REQ_DONE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING,
                    TILE_KERNELS, TWS_ERR_CAPACITY, BatchStats, DevicePlanes, Params, Request,
-                   RequestStatus, ServerConfig, ServerStats, TwsError, check, lib)
+                   RequestStatus, ServerConfig, ServerStats, TuneEntry, TwsError, check, lib)
 
 __all__ = [
     "Simulation", "Batch", "Server", "Params", "TwsError", "grid_footprint", "params_physical", "params_reference",
@@ -216,6 +216,26 @@ class Simulation:
         return p
 
 
+def autotune(shapes, steps: int = 100):
+    """Times every (variant, k) candidate on scratch grids of `shapes`
+    [(height, width), ...]; returns (best, all trials) as dicts."""
+    hs = np.ascontiguousarray([h for h, w in shapes], np.int32)
+    ws = np.ascontiguousarray([w for h, w in shapes], np.int32)
+    best = TuneEntry()
+    cap = 64
+    res = (TuneEntry * cap)()
+    cnt = C.c_int32()
+    check(lib().tws_autotune(_ptr(ws), _ptr(hs), len(shapes), int(steps), C.byref(best), res, cap,
+                             C.byref(cnt)))
+    conv = lambda e: {"kernel": e.kernel, "k": e.steps_per_launch,
+                      "cell_updates_per_s": e.cell_updates_per_s}
+    return conv(best), [conv(res[i]) for i in range(cnt.value)]
+
+
+def tune_forget():
+    check(lib().tws_tune_forget())
+
+
 def grid_footprint(width: int, height: int) -> int:
     """Device bytes of one grid's state (its admission footprint in a batch)."""
     return lib().tws_grid_footprint(int(width), int(height))
@@ -383,6 +403,10 @@ class Server:
         check(lib().tws_server_submit(self._h, C.byref(r), C.byref(ticket)))
         self._keep[ticket.value] = out
         return ticket.value, out
+
+    def release(self, ticket):
+        """Forgets the output arrays of a finished request."""
+        self._keep.pop(ticket, None)
 
     def poll(self, ticket) -> RequestStatus:
         st = RequestStatus()

@@ -76,6 +76,11 @@ class BatchStats(C.Structure):
     ]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("steps_per_launch", C.c_int32),
+                ("cell_updates_per_s", C.c_double)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("pool_bytes", C.c_int64),
@@ -163,6 +168,8 @@ PROTOTYPES = {
     "tws_batch_use_graphs": (C.c_int, [_vp, _i32]),
     "tws_batch_stats_get": (C.c_int, [_vp, C.POINTER(BatchStats)]),
     "tws_batch_sync": (C.c_int, [_vp]),
+    "tws_autotune": (C.c_int, [_vp, _vp, _i32, _i32, C.POINTER(TuneEntry), C.POINTER(TuneEntry), _i32, C.POINTER(_i32)]),
+    "tws_tune_forget": (C.c_int, []),
     "tws_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "tws_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
     "tws_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(RequestStatus)]),

@@ -1,21 +1,126 @@
-// tws_tune.cpp -- launch-configuration table for the fused step kernel.
+// tws_tune.cpp -- launch configuration of the fused step kernel: the default
+// table and the measuring autotuner.
 //
 // choose_launch() maps a grid size to (kernel variant, steps per launch).
-// The table starts from the design estimate (the 128 x 128 tile at 4 steps
-// per launch balances halo recomputation against HBM traffic) and is
-// overwritten by the autotuner's measurements (tws_autotune).
+// The built-in entry comes from the MI355X sweep (128 x 64 tiles, two work
+// groups per CU so one loads while the other computes, 4 steps per launch:
+// 3.2e11 cell-updates/s at 4096^2).  tws_autotune() times every candidate on
+// scratch grids of the requested shapes and records the winner for that
+// size class (log2 of the cell count), which choose_launch() then prefers.
 
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "tws.h"
 #include "tws_runtime.hpp"
 
 namespace tws {
 
+static std::mutex g_tune_mu;
+static std::map<int, std::pair<int, int>> g_tuned;  // size class -> (kernel, k)
+
+static int size_class(int64_t cells) {
+    int c = 0;
+    while (cells > 1) { cells >>= 1; ++c; }
+    return c;
+}
+
 void choose_launch(int width, int height, int* kernel, int* k) {
     const int64_t cells = (int64_t)width * height;
-    // 128 x 64 tiles (512 threads, two work groups per CU so one loads while
-    // the other computes) at 4 steps per launch: 3.2e11 cell-updates/s on a
-    // 4096^2 grid, the best of the (variant, k) sweep on MI355X.
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(size_class(cells));
+        if (it != g_tuned.end()) {
+            *kernel = it->second.first;
+            *k = it->second.second;
+            return;
+        }
+    }
     *kernel = TWS_KERNEL_TILE_128x64;
     *k = cells <= 256 * 256 ? 2 : 4;
 }
 
 }  // namespace tws
+
+using namespace tws;
+
+extern "C" {
+
+int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32_t steps,
+                 tws_tune_entry* best, tws_tune_entry* results, int32_t capacity,
+                 int32_t* count) {
+    if (!widths || !heights || n <= 0 || steps <= 0 || !best)
+        return fail(TWS_ERR_INVALID, "tws_autotune: invalid argument");
+    int64_t bytes = 0, cells = 0;
+    for (int i = 0; i < n; ++i) {
+        if (widths[i] <= 0 || heights[i] <= 0)
+            return fail(TWS_ERR_INVALID, "tws_autotune: invalid shape %d", i);
+        bytes += tws_grid_footprint(widths[i], heights[i]);
+        cells += (int64_t)widths[i] * heights[i];
+    }
+    tws_params p{};
+    int rc = tws_params_physical(0.02f, 9.81f, 1.0f, 1.0f, 0.995f, TWS_BOUNDARY_CLOSED, &p);
+    if (rc != TWS_OK) return rc;
+    tws_batch* b = nullptr;
+    rc = tws_batch_create(n, bytes, nullptr, &b);
+    if (rc != TWS_OK) return rc;
+    // synthetic wet state: fBm terrain 0..100 m under a 0.5 m sheet
+    std::vector<float> terrain, depth;
+    for (int i = 0; i < n && rc == TWS_OK; ++i) {
+        int32_t id = -1;
+        rc = tws_batch_add(b, widths[i], heights[i], &p, &id);
+        if (rc != TWS_OK) break;
+        const size_t c = (size_t)widths[i] * heights[i];
+        terrain.resize(c);
+        depth.assign(c, 0.5f);
+        rc = tws_generate_fbm_terrain(widths[i], heights[i], 1000u + (uint64_t)i, 6, 4, 0.43f,
+                                      100.0f, terrain.data());
+        if (rc == TWS_OK) rc = tws_batch_upload(b, id, terrain.data(), depth.data(), nullptr);
+    }
+    static const int kernels[] = {TWS_KERNEL_TILE_128x64, TWS_KERNEL_TILE_128x128,
+                                  TWS_KERNEL_TILE_256x64};
+    static const int ks[] = {2, 3, 4, 6, 8};
+    tws_tune_entry top{0, 0, 0.0};
+    int m = 0;
+    for (int kern : kernels) {
+        for (int kk : ks) {
+            if (rc != TWS_OK) break;
+            rc = tws_batch_set_launch(b, kern, kk);
+            if (rc == TWS_OK) rc = tws_batch_step(b, kk);  // warm: capture + first replay
+            if (rc == TWS_OK) rc = tws_batch_sync(b);
+            const auto t0 = std::chrono::steady_clock::now();
+            if (rc == TWS_OK) rc = tws_batch_step(b, steps);
+            if (rc == TWS_OK) rc = tws_batch_sync(b);
+            const double sec =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (rc != TWS_OK) break;
+            const tws_tune_entry e{kern, kk, (double)cells * steps / sec};
+            if (results && m < capacity) results[m] = e;
+            ++m;
+            if (e.cell_updates_per_s > top.cell_updates_per_s) top = e;
+        }
+    }
+    tws_batch_destroy(b);
+    if (rc != TWS_OK) return rc;
+    if (count) *count = m;
+    *best = top;
+    {
+        // record for the size class of the typical grid of the set
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[size_class(cells / n)] = {top.kernel, top.steps_per_launch};
+    }
+    return TWS_OK;
+}
+
+int tws_tune_forget(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+    return TWS_OK;
+}
+
+}  // extern "C"

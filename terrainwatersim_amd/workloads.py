@@ -1,0 +1,247 @@
+"""Benchmark workloads beside the headline plain-grid step (used by bench.py).
+
+batched  256 independent terrains, sides log-uniform 128..2048 (seed 3),
+         per-grid random rain discs (r in [8, 32], 1e-3 m/step), 100 steps per
+         captured launch sequence; (variant, k) from the autotuner.  Compared
+         with the same grids stepped one at a time.
+served   Poisson stream of solves, sides log-uniform 256..4096, steps uniform
+         100..5000, dam-break (20 m column over a centred square of side/8)
+         or rain (r = 16 at the centre, 1e-3 m/step) 50/50, arrival rate 70%
+         of the measured capacity; the continuous-batching server against the
+         plain path taking one request at a time in arrival order.  Latency is
+         arrival -> result on the host.
+large    16384^2 dam-break (20 m over the centred 2048^2 square) on fBm
+         terrain, closed walls, 10,000 steps: throughput and volume drift
+         (tests/test_large_gpu.py checks a 1024^2 window against the
+         restatements).
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+import threading
+import time
+
+import numpy as np
+
+import terrainwatersim_amd as tws
+
+F32 = np.float32
+PARAMS = dict(dt=0.02, gravity=9.81, cell_size=1.0, pipe_area=1.0, damping=0.995)
+
+
+def _splitmix(x: int) -> int:
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+# ---------------------------------------------------------------------------
+def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
+    rng = np.random.default_rng(seed)
+    sides = np.round(np.exp(rng.uniform(math.log(128), math.log(2048), n_grids))).astype(int)
+    p = tws.params_physical(**PARAMS)
+    grids = []
+    for i, s in enumerate(sides):
+        b = tws.generate_fbm_terrain(int(s), int(s), seed=_splitmix(seed * 1000003 + i))
+        r = float(rng.uniform(8, 32))
+        rain = (float(rng.uniform(0, s)), float(rng.uniform(0, s)), 1e-3, r * r)
+        grids.append((int(s), b, rain))
+    cells = int(sum(s * s for s, _, _ in grids))
+    log(f"batched: {n_grids} grids, {cells / 1e6:.1f} M cells; autotuning")
+    best, trials = tws.autotune([(s, s) for s, _, _ in grids], steps=50)
+    pool = sum(tws.grid_footprint(s, s) for s, _, _ in grids)
+    out = {"grids": n_grids, "cells": cells, "steps_per_run": steps,
+           "autotune_best": best, "autotune_trials": trials}
+    with tws.Batch(n_grids, pool) as batch:
+        ids = []
+        for s, b, rain in grids:
+            gid = batch.add(s, s, p)
+            batch.upload(gid, b, np.zeros((s, s), F32))
+            batch.set_rain(gid, *rain)
+            ids.append(gid)
+        batch.set_launch(best["kernel"], best["k"])
+        batch.step(steps)  # capture + warm
+        batch.sync()
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            batch.step(steps)
+            batch.sync()
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        out["batched_cell_updates_per_s"] = cells * steps / t
+        out["batched_ms_per_run"] = t * 1e3
+        out["graph"] = batch.stats()
+        checks = [batch.download(g, flux=False)["depth"] for g in ids[:4]]
+    # the same grids, one at a time (each stepped and synchronised alone)
+    log("batched: one grid at a time")
+    sims = []
+    try:
+        for s, b, rain in grids:
+            sim = tws.Simulation(s, s, p)
+            sim.set_launch(best["kernel"], best["k"])
+            sim.upload(b, np.zeros((s, s), F32))
+            sim.set_rain(*rain)
+            sims.append(sim)
+        for sim in sims:
+            sim.step(steps)
+            sim.sync()
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            for sim in sims:
+                sim.step(steps)
+                sim.sync()
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        out["plain_cell_updates_per_s"] = cells * steps / t
+        out["plain_ms_per_run"] = t * 1e3
+        same = all(np.array_equal(c, sim.download(flux=False)["depth"])
+                   for c, sim in zip(checks, sims[:4]))
+    finally:
+        for sim in sims:
+            sim.close()
+    out["batched_over_plain"] = out["batched_cell_updates_per_s"] / out["plain_cell_updates_per_s"]
+    out["batched_equals_plain"] = bool(same)
+    return out
+
+
+# ---------------------------------------------------------------------------
+class _Requests:
+    """The served request stream: terrain windows cut from one fBm field."""
+
+    def __init__(self, n, seed, rate):
+        rng = np.random.default_rng(seed)
+        self.sides = np.round(np.exp(rng.uniform(math.log(256), math.log(4096), n))).astype(int)
+        self.steps = rng.integers(100, 5001, n)
+        self.kind = rng.integers(0, 2, n)  # 0 dam-break, 1 rain
+        self.arrival = np.cumsum(rng.exponential(1.0 / rate, n))
+        self.arrival -= self.arrival[0]
+        self.offsets = rng.integers(0, 4096, (n, 2))
+        self.field = tws.generate_fbm_terrain(4096, 4096, seed=seed)
+
+    def state(self, i):
+        s = int(self.sides[i])
+        oy, ox = self.offsets[i]
+        ys = (np.arange(s) + oy) % 4096
+        xs = (np.arange(s) + ox) % 4096
+        b = np.ascontiguousarray(self.field[np.ix_(ys, xs)])
+        d = np.zeros((s, s), F32)
+        rain = None
+        if self.kind[i] == 0:
+            a = s // 8
+            c0 = (s - a) // 2
+            d[c0:c0 + a, c0:c0 + a] = 20.0
+        else:
+            rain = (s / 2.0, s / 2.0, 1e-3, 256.0)
+        return b, d, rain
+
+    def work(self):
+        return float(np.sum(self.sides.astype(np.float64) ** 2 * self.steps))
+
+
+def _pcts(lat):
+    lat = np.asarray(lat)
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}
+
+
+def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log=print):
+    p = tws.params_physical(**PARAMS)
+    probe = _Requests(n_requests, seed, 1.0)
+    mean_work = probe.work() / n_requests
+    rate = load * capacity_cups / mean_work
+    req = _Requests(n_requests, seed, rate)
+    log(f"served: {n_requests} requests, mean work {mean_work:.3g} cell-updates, "
+        f"rate {rate:.2f}/s ({load:.0%} of {capacity_cups:.3g} cell-updates/s)")
+    states = [req.state(i) for i in range(n_requests)]
+    pool = int(min(96 << 30, 40 * tws.grid_footprint(4096, 4096)))
+    digests_served = {}
+    lat_served = []
+    with tws.Server(pool, max_resident=64, chunk_steps=chunk_steps) as srv:
+        tickets = {}
+        t0 = time.perf_counter()
+        base = srv.now()
+        for i in range(n_requests):
+            wait = req.arrival[i] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            b, d, rain = states[i]
+            t, out = srv.submit(b, int(req.steps[i]), p, depth=d, rain=rain, want_flux=False)
+            tickets[t] = (i, out)
+        srv.drain()
+        t_end = 0.0
+        for t, (i, out) in tickets.items():
+            st = srv.poll(t)
+            assert st.state == tws.REQ_DONE, (i, st.state, st.error)
+            lat_served.append(st.t_done - st.t_submit)
+            t_end = max(t_end, st.t_done - base)
+            digests_served[i] = hashlib.sha1(out["depth"].tobytes()).hexdigest()
+            srv.release(t)
+        stats = srv.stats()
+    work = req.work()
+    res = {"requests": n_requests, "arrival_rate_per_s": rate, "load": load,
+           "chunk_steps": chunk_steps, "cell_updates": work,
+           "served": {"cell_updates_per_s": work / t_end, "makespan_s": t_end,
+                      **_pcts(lat_served), "batched_runs": stats["runs"],
+                      "gpu_busy_s": stats["busy_seconds"]}}
+    log(f"served: done in {t_end:.2f}s; plain one-at-a-time")
+    lat_plain, same = [], True
+    t0 = time.perf_counter()
+    for i in range(n_requests):
+        wait = req.arrival[i] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        b, d, rain = states[i]
+        s = b.shape[0]
+        with tws.Simulation(s, s, p) as sim:
+            sim.upload(b, d)
+            if rain:
+                sim.set_rain(*rain)
+            sim.step(int(req.steps[i]))
+            dd = sim.download(flux=False)["depth"]
+        lat_plain.append(time.perf_counter() - t0 - req.arrival[i])
+        same &= hashlib.sha1(dd.tobytes()).hexdigest() == digests_served[i]
+    t_plain = time.perf_counter() - t0
+    res["plain"] = {"cell_updates_per_s": work / t_plain, "makespan_s": t_plain,
+                    **_pcts(lat_plain)}
+    res["served_over_plain"] = res["served"]["cell_updates_per_s"] / res["plain"]["cell_updates_per_s"]
+    res["served_equals_plain"] = bool(same)
+    return res
+
+
+# ---------------------------------------------------------------------------
+def large_state(N=16384, seed=5):
+    """fBm terrain and the dam-break depth: 20 m over the centred 2048^2 square."""
+    b = tws.generate_fbm_terrain(N, N, seed=seed)
+    d = np.zeros((N, N), F32)
+    c0 = (N - 2048) // 2
+    d[c0:c0 + 2048, c0:c0 + 2048] = 20.0
+    return b, d
+
+
+def large(steps=10000, N=16384, seed=5, log=print):
+    p = tws.params_physical(**PARAMS)
+    log(f"large: {N}^2 fBm terrain, dam-break")
+    b, d = large_state(N, seed)
+    out = {"grid": N, "steps": steps, "footprint_bytes": tws.grid_footprint(N, N)}
+    with tws.Simulation(N, N, p) as sim:
+        sim.upload(b, d)
+        del b, d
+        v0 = sim.volume()
+        sim.step(50)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(steps - 50)
+        sim.sync()
+        el = time.perf_counter() - t0
+        v1 = sim.volume()
+        ok = sim.check_finite()
+        kernel, k = sim.get_launch()
+    out.update({"cell_updates_per_s": N * N * (steps - 50) / el, "seconds": el,
+                "kernel": kernel, "steps_per_launch": k, "finite": ok,
+                "volume_initial_m3": v0, "volume_final_m3": v1,
+                "volume_rel_drift": abs(v1 - v0) / v0})
+    return out

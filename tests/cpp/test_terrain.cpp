@@ -1,0 +1,84 @@
+// Drives the C++ mirror of the reference's Terrain class (include/tws_terrain.hpp)
+// and checks it against the same steps issued through the C ABI directly.
+// Built by __graft_entry__.build(); run by tests/test_terrain_cpp.py.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "tws_terrain.hpp"
+
+static int failures = 0;
+#define EXPECT(c)                                                    \
+    do {                                                             \
+        if (!(c)) { std::printf("FAILED: %s (line %d)\n", #c, __LINE__); ++failures; } \
+    } while (0)
+
+int main() {
+    const int N = 256;
+    tws::Terrain terrain(N, 1024.0f);
+    EXPECT(terrain.Ok());
+    // Terrain.h:76 returns (float)(1/step + 0.5): 60.5 at 60 steps/s, as in the reference
+    EXPECT(terrain.GetSimulationStepsPerSecond() == 60.5f);
+    EXPECT(terrain.GetFlowDamping() == 0.98f && terrain.GetFlowAcceleration() == 10.0f);
+
+    // initial state, captured through the reference layout
+    std::vector<float> t0, f0;
+    terrain.ReadReferenceTextures(&t0, &f0, nullptr);
+    std::vector<float> b(N * N), d(N * N);
+    for (int i = 0; i < N * N; ++i) { b[i] = t0[i * 4 + 0]; d[i] = t0[i * 4 + 3]; }
+    // default lake: water only inside 800 r^4 < 0.45 (r from the grid centre)
+    double lake = 0;
+    int outside_wet = 0;
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+            const double dx = x / double(N - 1) - 0.5, dy = y / double(N - 1) - 0.5;
+            const double r2 = dx * dx + dy * dy;
+            lake += d[y * N + x];
+            if (800.0 * r2 * r2 > 0.4501 && d[y * N + x] > 0.0f) ++outside_wet;
+        }
+    EXPECT(lake > 0.0);
+    EXPECT(outside_wet == 0);
+
+    // 51 ms at 60 steps/s: 3 steps (the step length is the float 1/60 held as
+    // double, 0.01666666754 s, so exactly 50 ms would give only 2); then a
+    // brush; then a 1 s frame, clamped to 10 steps
+    terrain.PerformSimulationStep(0.051);
+    EXPECT(terrain.LastStepCount() == 3);
+    terrain.ApplyRadialWaterBrush(1024.0f + 100.0f, -20.0f, 2.0f);  // wraps into the grid
+    terrain.PerformSimulationStep(1.0);
+    EXPECT(terrain.LastStepCount() == 10);
+    terrain.SetFlowDamping(0.9f);
+    terrain.PerformSimulationStep(1.0 / 60.0 + 1e-9);
+    EXPECT(terrain.LastStepCount() == 1);
+
+    // the same through the C ABI
+    tws_params p{};
+    EXPECT(tws_params_reference(60.0f, 0.98f, 10.0f, 1024.0f, N, TWS_BOUNDARY_CLOSED, &p) == TWS_OK);
+    tws_sim* s = nullptr;
+    EXPECT(tws_create(N, N, &p, nullptr, &s) == TWS_OK);
+    EXPECT(tws_upload(s, b.data(), d.data(), nullptr) == TWS_OK);
+    EXPECT(tws_step(s, 3) == TWS_OK);
+    const float u = 100.0f / 1024.0f, v = 1.0f - 20.0f / 1024.0f;
+    EXPECT(tws_apply_brush(s, u * N, (v - 0.0f) * N, 2.0f, 32.0f) == TWS_OK);
+    EXPECT(tws_step(s, 10) == TWS_OK);
+    tws_params p2{};
+    EXPECT(tws_params_reference(60.0f, 0.9f, 10.0f, 1024.0f, N, TWS_BOUNDARY_CLOSED, &p2) == TWS_OK);
+    EXPECT(tws_set_params(s, &p2) == TWS_OK);
+    EXPECT(tws_step(s, 1) == TWS_OK);
+    std::vector<float> dd(N * N), ff(N * N * 4);
+    EXPECT(tws_download(s, dd.data(), ff.data(), nullptr) == TWS_OK);
+    tws_destroy(s);
+
+    std::vector<float> t1, f1;
+    terrain.ReadReferenceTextures(&t1, &f1, nullptr);
+    int diff = 0;
+    for (int i = 0; i < N * N; ++i) {
+        if (t1[i * 4 + 3] != dd[i]) ++diff;
+        for (int c = 0; c < 4; ++c)
+            if (f1[i * 4 + c] != ff[i * 4 + c]) ++diff;
+    }
+    EXPECT(diff == 0);
+    EXPECT(terrain.Ok());
+    std::printf(failures ? "FAILED %d\n" : "OK\n", failures);
+    return failures ? 1 : 0;
+}

@@ -187,6 +187,11 @@ int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
  * 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */
 int tws_set_launch(tws_sim* sim, int32_t kernel, int32_t steps_per_launch);
 int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launch);
+/* Order in which the step kernel walks a grid's tiles: groups of
+ * `group_columns` tile columns, row by row inside a group, so the tiles in
+ * flight together form a compact block that shares its halos in L2
+ * (0 = row-major).  Results do not depend on it. */
+int tws_set_tile_order(tws_sim* sim, int32_t group_columns);
 
 /* Host-side float -> IEEE half conversion rounding toward zero, the FlowMap
  * rounding of tws_download_reference_layout, for n values. */
@@ -373,6 +378,22 @@ int tws_server_destroy(tws_server* server);
 int tws_generate_fbm_terrain(int32_t width, int32_t height, uint64_t seed, int32_t octaves,
                              int32_t base_frequency, float persistence, float height_scale,
                              float* out);
+
+/* The same fBm computed on the device straight into a simulation's terrain
+ * plane (bit-identical to tws_generate_fbm_terrain), and a device-side water
+ * initialiser: depth `depth` everywhere except rect_depth inside
+ * [x0, x1) x [y0, y1) (an empty rectangle for a uniform sheet; a column for
+ * a dam break), flux zero.  Replace the host loops of
+ * Terrain::CreateHeightmapFromNoiseAndResetSim (Terrain.cpp:209-234) with no
+ * host-to-device traffic.  Asynchronous on the handle's stream. */
+int tws_init_terrain_fbm(tws_sim* sim, uint64_t seed, int32_t octaves, int32_t base_frequency,
+                         float persistence, float height_scale);
+int tws_init_water(tws_sim* sim, float depth, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                   float rect_depth);
+int tws_batch_init_terrain_fbm(tws_batch* batch, int32_t grid_id, uint64_t seed, int32_t octaves,
+                               int32_t base_frequency, float persistence, float height_scale);
+int tws_batch_init_water(tws_batch* batch, int32_t grid_id, float depth, int32_t x0, int32_t y0,
+                         int32_t x1, int32_t y1, float rect_depth);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 

@@ -195,6 +195,18 @@ class Simulation:
         check(lib().tws_get_launch(self._h, C.byref(k), C.byref(s)))
         return k.value, s.value
 
+    def init_terrain_fbm(self, seed: int, octaves: int = 6, base_frequency: int = 4,
+                         persistence: float = 0.43, height_scale: float = 100.0):
+        check(lib().tws_init_terrain_fbm(self._h, int(seed), octaves, base_frequency, persistence,
+                                         height_scale))
+
+    def init_water(self, depth: float = 0.0, rect=(0, 0, 0, 0), rect_depth: float = 0.0):
+        x0, y0, x1, y1 = rect
+        check(lib().tws_init_water(self._h, depth, x0, y0, x1, y1, rect_depth))
+
+    def set_tile_order(self, group_columns: int):
+        check(lib().tws_set_tile_order(self._h, int(group_columns)))
+
     def sync(self):
         check(lib().tws_sync(self._h))
 
@@ -303,6 +315,15 @@ class Batch:
 
     def set_params(self, gid, p: Params):
         check(lib().tws_batch_set_params(self._h, int(gid), C.byref(p)))
+
+    def init_terrain_fbm(self, gid, seed: int, octaves: int = 6, base_frequency: int = 4,
+                         persistence: float = 0.43, height_scale: float = 100.0):
+        check(lib().tws_batch_init_terrain_fbm(self._h, int(gid), int(seed), octaves,
+                                               base_frequency, persistence, height_scale))
+
+    def init_water(self, gid, depth: float = 0.0, rect=(0, 0, 0, 0), rect_depth: float = 0.0):
+        x0, y0, x1, y1 = rect
+        check(lib().tws_batch_init_water(self._h, int(gid), depth, x0, y0, x1, y1, rect_depth))
 
     def set_rain(self, gid, cx, cy, intensity, radius_sq):
         check(lib().tws_batch_set_rain(self._h, int(gid), cx, cy, intensity, radius_sq))

@@ -145,7 +145,12 @@ PROTOTYPES = {
     "tws_get_device_planes": (C.c_int, [_vp, C.POINTER(DevicePlanes)]),
     "tws_set_launch": (C.c_int, [_vp, _i32, _i32]),
     "tws_get_launch": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32)]),
+    "tws_set_tile_order": (C.c_int, [_vp, _i32]),
     "tws_generate_fbm_terrain": (C.c_int, [_i32, _i32, C.c_uint64, _i32, _i32, C.c_float, C.c_float, _vp]),
+    "tws_init_terrain_fbm": (C.c_int, [_vp, C.c_uint64, _i32, _i32, C.c_float, C.c_float]),
+    "tws_init_water": (C.c_int, [_vp, C.c_float, _i32, _i32, _i32, _i32, C.c_float]),
+    "tws_batch_init_terrain_fbm": (C.c_int, [_vp, _i32, C.c_uint64, _i32, _i32, C.c_float, C.c_float]),
+    "tws_batch_init_water": (C.c_int, [_vp, _i32, C.c_float, _i32, _i32, _i32, _i32, C.c_float]),
     "tws_debug_lane_shift_probe": (C.c_int, [C.POINTER(_i32)]),
     "tws_float_to_half_rtz": (C.c_int, [_vp, _vp, C.c_int64]),
     "tws_stream_copy_bench": (C.c_int, [C.c_int64, _i32, C.POINTER(C.c_double)]),

@@ -232,6 +232,21 @@ int tws_batch_download(tws_batch* b, int32_t id, float* depth, float* flux4, flo
     return grid_download(s->g, depth, flux4, velocity2, b->stream);
 }
 
+int tws_batch_init_terrain_fbm(tws_batch* b, int32_t id, uint64_t seed, int32_t octaves,
+                               int32_t base_frequency, float persistence, float height_scale) {
+    Slot* s = slot_of(b, id);
+    if (!s) return fail(TWS_ERR_INVALID, "tws_batch_init_terrain_fbm: unknown grid %d", id);
+    return grid_init_terrain_fbm(s->g, seed, octaves, base_frequency, persistence, height_scale,
+                                 b->stream);
+}
+
+int tws_batch_init_water(tws_batch* b, int32_t id, float depth, int32_t x0, int32_t y0,
+                         int32_t x1, int32_t y1, float rect_depth) {
+    Slot* s = slot_of(b, id);
+    if (!s) return fail(TWS_ERR_INVALID, "tws_batch_init_water: unknown grid %d", id);
+    return grid_init_water(s->g, depth, x0, y0, x1, y1, rect_depth, b->stream);
+}
+
 int tws_batch_set_params(tws_batch* b, int32_t id, const tws_params* p) {
     Slot* s = slot_of(b, id);
     if (!s) return fail(TWS_ERR_INVALID, "tws_batch_set_params: unknown grid %d", id);

@@ -4,6 +4,7 @@
 
 #include <cmath>
 
+#include "tws_fbm.hpp"
 #include "tws_grid.hpp"
 #include "tws_runtime.hpp"
 
@@ -46,6 +47,7 @@ GridDesc GridState::desc() const {
     g.rain_R2 = rain[3];
     g.status = status;
     g.parity0 = parity;
+    g.tile_group = tile_group;
     return g;
 }
 
@@ -193,6 +195,24 @@ int grid_volume(GridState& g, double* partials_dev, double* out, hipStream_t s) 
     double acc = 0.0;
     for (int i = 0; i < kVolumeParts; ++i) acc += host[i];
     *out = acc;
+    return TWS_OK;
+}
+
+int grid_init_terrain_fbm(GridState& g, uint64_t seed, int octaves, int base, float persistence,
+                          float scale, hipStream_t s) {
+    if (!fbm_args_ok(g.W, g.H, octaves, base))
+        return fail(TWS_ERR_INVALID, "init_terrain_fbm: invalid octaves/base frequency");
+    TWS_HIP(launch_fbm(g.b, g.desc(), seed, octaves, base, (double)persistence, (double)scale, s),
+            "init_terrain_fbm");
+    return TWS_OK;
+}
+
+int grid_init_water(GridState& g, float depth, int x0, int y0, int x1, int y1, float rect_depth,
+                    hipStream_t s) {
+    if (!std::isfinite(depth) || !std::isfinite(rect_depth))
+        return fail(TWS_ERR_INVALID, "init_water: non-finite depth");
+    TWS_HIP(launch_init_water(g.desc(), g.parity, depth, x0, y0, x1, y1, rect_depth, s),
+            "init_water");
     return TWS_OK;
 }
 

@@ -30,6 +30,7 @@ struct GridState {
     int parity = 0;
     int rain_on = 0;
     float rain[4] = {0, 0, 0, 0};
+    int tile_group = 0;          // tile order of the step kernel (0 = row-major)
 
     static size_t bytes_for(int W, int H) {
         return sizeof(float) * (size_t)kPlanes * (size_t)pitch_of(W) * (size_t)H;
@@ -52,6 +53,10 @@ int grid_brush(GridState& g, float cx, float cy, float I, float R2, hipStream_t 
 int grid_set_rain(GridState& g, float cx, float cy, float I, float R2);
 int grid_volume(GridState& g, double* partials_dev, double* out, hipStream_t s);
 int grid_check_finite(GridState& g, hipStream_t s);  // reads and clears the flag
+int grid_init_terrain_fbm(GridState& g, uint64_t seed, int octaves, int base, float persistence,
+                          float scale, hipStream_t s);
+int grid_init_water(GridState& g, float depth, int x0, int y0, int x1, int y1, float rect_depth,
+                    hipStream_t s);
 
 // Tiles of a grid for a (kernel variant, steps per launch) pair.
 bool grid_tiles(int W, int H, int kernel, int k, int* tiles_x, int* tiles_y);

@@ -50,7 +50,7 @@ struct GridDesc {
     float ghost_b, ghost_d, ghost_fw;  // ghost cell: terrain, depth, -y flux
     float rain_cx, rain_cy, rain_I, rain_R2;
     int32_t rain_on;
-    int32_t pad_;
+    int32_t tile_group;      // tile order: columns per group (0 = row-major)
     unsigned int* status;    // bit 0: non-finite value seen
 };
 
@@ -97,6 +97,10 @@ hipError_t launch_volume_partials(const GridDesc& g, int parity, double* partial
 hipError_t launch_zero_fill(float* p, int64_t n, hipStream_t s);
 hipError_t launch_lane_shift_probe(int* out128, hipStream_t s);
 hipError_t launch_check_finite(const GridDesc& g, int parity, hipStream_t s);
+hipError_t launch_fbm(float* b, const GridDesc& g, uint64_t seed, int octaves, int base,
+                      double persistence, double scale, hipStream_t s);
+hipError_t launch_init_water(const GridDesc& g, int parity, float depth, int x0, int y0, int x1,
+                             int y1, float rect_depth, hipStream_t s);
 hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 constexpr int kVolumeParts = 512;

@@ -22,6 +22,7 @@
 
 #include "tws_internal.hpp"
 #include "tws.h"
+#include "tws_fbm.hpp"
 
 namespace tws {
 
@@ -165,8 +166,21 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ 
     if (levels <= 0) return;
     const int parity = (g.parity0 + launch_idx) & 1;
     const int lt = tile - g.tile_begin;
-    const int ty = lt / g.tiles_x;
-    const int tx = lt - ty * g.tiles_x;
+    // Tile order inside the grid: groups of tile_group columns, row by row
+    // within a group, so the tiles in flight at once form a compact block
+    // whose shared halos meet in L2 (tile_group 0: plain row-major).
+    int tx, ty;
+    if (g.tile_group > 0) {
+        const int per_group = g.tile_group * g.tiles_y;
+        const int grp = lt / per_group, in_grp = lt - grp * per_group;
+        const int first = grp * g.tile_group;
+        const int gw = min(g.tiles_x - first, g.tile_group);
+        ty = in_grp / gw;
+        tx = first + (in_grp - ty * gw);
+    } else {
+        ty = lt / g.tiles_x;
+        tx = lt - ty * g.tiles_x;
+    }
     const int ox = tx * G::OX - G::HALOX;
     const int oy = ty * G::OY - G::HALOY;
     const int W = g.W, H = g.H;
@@ -550,6 +564,30 @@ __global__ void __launch_bounds__(256) k_check_finite(GridDesc g, int parity) {
     if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(g.status, 1u);
 }
 
+// Seeded fBm terrain straight into the b plane (same bits as the host
+// generator: tws_fbm.hpp).
+__global__ void __launch_bounds__(256) k_fbm(float* b, int64_t pitch, int W, int H,
+                                              uint64_t seed, int octaves, int base,
+                                              double persistence, double scale) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    b[(int64_t)y * pitch + x] = fbm_cell(x, y, W, H, seed, octaves, base, persistence, scale);
+}
+
+// Water of set `parity`: rect_depth inside [x0, x1) x [y0, y1), `depth`
+// elsewhere; flux zero.
+__global__ void __launch_bounds__(256) k_init_water(GridDesc g, int parity, float depth, int x0,
+                                                     int y0, int x1, int y1, float rect_depth) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= g.W || y >= g.H) return;
+    const int64_t i = (int64_t)y * g.pitch + x;
+    const bool in = x >= x0 && x < x1 && y >= y0 && y < y1;
+    g.d[parity][i] = in ? rect_depth : depth;
+    for (int c = 0; c < 4; ++c) g.f[parity][c][i] = 0.0f;
+}
+
 // Plain device copy of n float4 (STREAM copy): the measured HBM peak, and
 // the known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters.
 __global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src,
@@ -694,6 +732,20 @@ hipError_t launch_zero_fill(float* p, int64_t n, hipStream_t s) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_zero_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fbm(float* b, const GridDesc& g, uint64_t seed, int octaves, int base,
+                      double persistence, double scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_fbm, cell_grid(g), dim3(256), 0, s, b, g.pitch, g.W, g.H, seed, octaves,
+                       base, persistence, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_water(const GridDesc& g, int parity, float depth, int x0, int y0, int x1,
+                             int y1, float rect_depth, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_water, cell_grid(g), dim3(256), 0, s, g, parity, depth, x0, y0, x1,
+                       y1, rect_depth);
     return hipGetLastError();
 }
 

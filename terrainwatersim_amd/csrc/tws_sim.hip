@@ -265,6 +265,25 @@ int tws_set_launch(tws_sim* s, int32_t kernel, int32_t k) {
     return TWS_OK;
 }
 
+int tws_init_terrain_fbm(tws_sim* s, uint64_t seed, int32_t octaves, int32_t base_frequency,
+                         float persistence, float height_scale) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_init_terrain_fbm: NULL handle");
+    return grid_init_terrain_fbm(s->g, seed, octaves, base_frequency, persistence, height_scale,
+                                 s->stream);
+}
+
+int tws_init_water(tws_sim* s, float depth, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                   float rect_depth) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_init_water: NULL handle");
+    return grid_init_water(s->g, depth, x0, y0, x1, y1, rect_depth, s->stream);
+}
+
+int tws_set_tile_order(tws_sim* s, int32_t group_columns) {
+    if (!s || group_columns < 0) return fail(TWS_ERR_INVALID, "tws_set_tile_order: bad argument");
+    s->g.tile_group = group_columns;
+    return TWS_OK;
+}
+
 int tws_get_launch(const tws_sim* s, int32_t* kernel, int32_t* k) {
     if (!s) return fail(TWS_ERR_INVALID, "tws_get_launch: NULL handle");
     if (kernel) *kernel = s->kernel;

@@ -1,73 +1,22 @@
-// tws_terrain_gen.cpp -- seeded value-noise fBm heightfield for initial states.
-//
-// The reference builds its terrain from a 3-D value-noise fBm with a quintic
-// fade over a period-wrapped lattice (NoiseGenerator.cpp:12-97, fade at
-// NoiseGenerator.h:22; persistence 0.43 at Terrain.cpp:216).  This generator
-// keeps those choices -- quintic fade, lattice wrapped to the domain,
-// persistence per octave -- in 2-D, with lattice values from splitmix64 of
-// (seed, octave, i, j) so that any size is reproducible from the seed alone.
-// It does not reproduce the reference's PRNG (Random.cpp); both sides of every
-// comparison receive the same arrays.
-//
-// Octave o has base_frequency * 2^o lattice cells across the domain and
-// amplitude persistence^o; the sum is divided by the total amplitude (range
-// [-1, 1]), mapped to [0, 1] and scaled by height_scale.  All arithmetic is
-// double (+, -, *, floor only) and the result is rounded once to float, so the
-// output is identical on every host.
+// tws_terrain_gen.cpp -- host side of the seeded fBm heightfield (the cell
+// function is shared with the device initialiser: tws_fbm.hpp).
 
 #include <algorithm>
-#include <cmath>
 #include <cstdint>
 #include <thread>
 #include <vector>
 
+#include "tws_fbm.hpp"
 #include "tws_runtime.hpp"
 
 namespace {
 
-inline uint64_t splitmix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
-
-// Lattice value in [-1, 1).
-inline double lattice(uint64_t seed, int octave, int64_t i, int64_t j) {
-    const uint64_t key = ((uint64_t)octave << 56) ^ ((uint64_t)i << 28) ^ (uint64_t)j;
-    const uint64_t h = splitmix64(seed ^ splitmix64(key));
-    return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-}
-
-inline double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
-
 void fbm_rows(int W, int H, uint64_t seed, int octaves, int base, double persistence,
               double scale, float* out, int y0, int y1) {
-    double amp_total = 0.0, amp = 1.0;
-    for (int o = 0; o < octaves; ++o) { amp_total += amp; amp *= persistence; }
-    for (int y = y0; y < y1; ++y) {
-        for (int x = 0; x < W; ++x) {
-            double sum = 0.0;
-            amp = 1.0;
-            for (int o = 0; o < octaves; ++o) {
-                const int64_t freq = (int64_t)base << o;
-                const double u = (double)x * (double)freq / (double)W;
-                const double v = (double)y * (double)freq / (double)H;
-                const double fu = std::floor(u), fv = std::floor(v);
-                const int64_t i0 = (int64_t)fu % freq, j0 = (int64_t)fv % freq;
-                const int64_t i1 = (i0 + 1) % freq, j1 = (j0 + 1) % freq;
-                const double su = fade(u - fu), sv = fade(v - fv);
-                const double a = lattice(seed, o, i0, j0), b = lattice(seed, o, i1, j0);
-                const double c = lattice(seed, o, i0, j1), d = lattice(seed, o, i1, j1);
-                const double ab = a + (b - a) * su;
-                const double cd = c + (d - c) * su;
-                sum += amp * (ab + (cd - ab) * sv);
-                amp *= persistence;
-            }
-            const double n = sum / amp_total;
-            out[(int64_t)y * W + x] = (float)((n * 0.5 + 0.5) * scale);
-        }
-    }
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < W; ++x)
+            out[(int64_t)y * W + x] = tws::fbm_cell(x, y, W, H, seed, octaves, base, persistence,
+                                                    scale);
 }
 
 }  // namespace
@@ -75,8 +24,7 @@ void fbm_rows(int W, int H, uint64_t seed, int octaves, int base, double persist
 extern "C" int tws_generate_fbm_terrain(int32_t width, int32_t height, uint64_t seed,
                                         int32_t octaves, int32_t base_frequency,
                                         float persistence, float height_scale, float* out) {
-    if (!out || width <= 0 || height <= 0 || octaves <= 0 || octaves > 24 ||
-        base_frequency <= 0 || ((int64_t)base_frequency << (octaves - 1)) > (1ll << 26))
+    if (!out || !tws::fbm_args_ok(width, height, octaves, base_frequency))
         return tws::fail(TWS_ERR_INVALID, "tws_generate_fbm_terrain: invalid arguments");
     const int64_t cells = (int64_t)width * height;
     unsigned nt = std::thread::hardware_concurrency();

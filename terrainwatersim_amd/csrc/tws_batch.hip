@@ -73,6 +73,17 @@ static Slot* slot_of(tws_batch* b, int32_t id) {
     return &b->slots[id];
 }
 
+namespace tws {
+// Device state of a batch grid, for library code that drives transfers on a
+// stream of its own (the server overlaps uploads and downloads of grids that
+// are not in the running launch sequence with that sequence).
+GridState* batch_grid(tws_batch* b, int32_t id) {
+    Slot* s = slot_of(b, id);
+    return s ? &s->g : nullptr;
+}
+hipStream_t batch_stream(tws_batch* b) { return b ? b->stream : nullptr; }
+}  // namespace tws
+
 static void drop_graphs(tws_batch* b) {
     for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
     b->graphs.clear();

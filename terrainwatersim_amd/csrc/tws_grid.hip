@@ -89,6 +89,26 @@ int grid_upload(GridState& g, const float* terrain, const float* depth, const fl
     return TWS_OK;
 }
 
+int grid_upload_async(GridState& g, const float* terrain, const float* depth, hipStream_t s) {
+    const size_t row = sizeof(float) * (size_t)g.W, dpitch = sizeof(float) * (size_t)g.pitch;
+    if (terrain)
+        TWS_HIP(hipMemcpy2DAsync(g.b, dpitch, terrain, row, row, g.H, hipMemcpyHostToDevice, s),
+                "upload: terrain");
+    if (depth)
+        TWS_HIP(hipMemcpy2DAsync(g.d[g.parity], dpitch, depth, row, row, g.H,
+                                 hipMemcpyHostToDevice, s), "upload: depth");
+    TWS_HIP(launch_zero_fill(g.f[g.parity][0], 4 * g.pitch * (int64_t)g.H, s), "upload: flux");
+    TWS_HIP(launch_check_finite(g.desc(), g.parity, s), "upload: finite check");
+    return TWS_OK;
+}
+
+int grid_download_depth_async(GridState& g, float* depth, hipStream_t s) {
+    const size_t row = sizeof(float) * (size_t)g.W, spitch = sizeof(float) * (size_t)g.pitch;
+    TWS_HIP(hipMemcpy2DAsync(depth, row, g.d[g.parity], spitch, row, g.H, hipMemcpyDeviceToHost,
+                             s), "download: depth");
+    return TWS_OK;
+}
+
 int grid_download(GridState& g, float* depth, float* flux4, float* velocity2, hipStream_t s) {
     const size_t row = sizeof(float) * (size_t)g.W, spitch = sizeof(float) * (size_t)g.pitch;
     const GridDesc gd = g.desc();

@@ -30,7 +30,9 @@ struct GridState {
     int parity = 0;
     int rain_on = 0;
     float rain[4] = {0, 0, 0, 0};
-    int tile_group = 0;          // tile order of the step kernel (0 = row-major)
+    // tile order of the step kernel: groups of 16 tile columns (0 = row-major);
+    // 16 measured best at 4096^2 (+1.6% at k = 4, +2.8% at k = 6)
+    int tile_group = 16;
 
     static size_t bytes_for(int W, int H) {
         return sizeof(float) * (size_t)kPlanes * (size_t)pitch_of(W) * (size_t)H;
@@ -60,5 +62,17 @@ int grid_init_water(GridState& g, float depth, int x0, int y0, int x1, int y1, f
 
 // Tiles of a grid for a (kernel variant, steps per launch) pair.
 bool grid_tiles(int W, int H, int kernel, int k, int* tiles_x, int* tiles_y);
+
+// Asynchronous transfers for callers that synchronise themselves: the host
+// buffers must be pinned (hipHostMalloc) for the copies to overlap anything.
+// Upload: terrain and depth (either may be NULL), flux reset to zero.
+int grid_upload_async(GridState& g, const float* terrain, const float* depth, hipStream_t s);
+// Download of the depth plane into a dense host buffer.
+int grid_download_depth_async(GridState& g, float* depth, hipStream_t s);
+
+// Device state of grid `id` of a batch (null if unknown), and the batch's
+// stream (where its launch sequences and grid clears run).
+GridState* batch_grid(tws_batch* b, int32_t id);
+hipStream_t batch_stream(tws_batch* b);
 
 }  // namespace tws

@@ -36,17 +36,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(b, d, params, target_s=10.0):
+def cpu_baseline(b, d, params, target_s=12.0):
     """FP32 restatement on all host cores, on a bounded number of steps of
-    the same grid (calibrated from one step to about target_s seconds)."""
+    the same grid (step count calibrated from two warm steps to about
+    target_s seconds)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import tws_oracle
     threads = tws_oracle.set_threads(min(os.cpu_count() or 1, 16))
     F, K, Cc = params
+    dd, ff, _ = tws_oracle.step(b, d, None, F, K, Cc, n=1)  # warm (threads, pages)
     t0 = time.perf_counter()
-    dd, ff, _ = tws_oracle.step(b, d, None, F, K, Cc, n=1)
-    one = time.perf_counter() - t0
-    steps = int(max(1, min(400, target_s / max(one, 1e-6))))
+    dd, ff, _ = tws_oracle.step(b, dd, ff, F, K, Cc, n=2)
+    one = (time.perf_counter() - t0) / 2
+    steps = int(max(1, min(1000, target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     tws_oracle.step(b, dd, ff, F, K, Cc, n=steps)
     el = time.perf_counter() - t0
@@ -56,7 +58,7 @@ def cpu_baseline(b, d, params, target_s=10.0):
         "unit": "cell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{W}x{H} grid, {steps} steps after 1 warm step ({el:.1f} s), "
+        "sample": f"{W}x{H} grid, {steps} steps after 3 warm steps ({el:.1f} s), "
                   "oracle/tws_oracle.c FP32 restatement, OpenMP over rows",
     }
 
@@ -70,7 +72,7 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="TWS_KERNEL_* (0 = auto)")
     ap.add_argument("--k", type=int, default=0, help="steps per launch (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB STREAM copy kernel (reference for PMC byte counts)")
     ap.add_argument("--extra", default="batched,served",

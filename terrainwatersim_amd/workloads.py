@@ -9,7 +9,8 @@ served   Poisson stream of solves, sides log-uniform 256..4096, steps uniform
          or rain (r = 16 at the centre, 1e-3 m/step) 50/50, arrival rate 70%
          of the measured capacity; the continuous-batching server against the
          plain path taking one request at a time in arrival order.  Latency is
-         arrival -> result on the host.
+         arrival -> result on the host.  A second server pass submits all
+         requests at once: its rate is the server's sustained capacity.
 large    16384^2 dam-break (20 m over the centred 2048^2 square) on fBm
          terrain, closed walls, 10,000 steps: throughput and volume drift
          (tests/test_large_gpu.py checks a 1024^2 window against the
@@ -187,7 +188,21 @@ def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log
            "served": {"cell_updates_per_s": work / t_end, "makespan_s": t_end,
                       **_pcts(lat_served), "batched_runs": stats["runs"],
                       "gpu_busy_s": stats["busy_seconds"]}}
-    log(f"served: done in {t_end:.2f}s; plain one-at-a-time")
+    # capacity: the same requests all at once (the server saturated)
+    log(f"served: done in {t_end:.2f}s; saturated pass")
+    with tws.Server(pool, max_resident=64, chunk_steps=chunk_steps) as srv:
+        base = srv.now()
+        tickets = []
+        for i in range(n_requests):
+            b, d, rain = states[i]
+            tickets.append(srv.submit(b, int(req.steps[i]), p, depth=d, rain=rain,
+                                      want_flux=False)[0])
+        srv.drain()
+        t_sat = max(srv.poll(t).t_done for t in tickets) - base
+        st = srv.stats()
+    res["served_saturated"] = {"cell_updates_per_s": work / t_sat, "makespan_s": t_sat,
+                               "gpu_busy_s": st["busy_seconds"]}
+    log("served: plain one-at-a-time")
     lat_plain, same = [], True
     t0 = time.perf_counter()
     for i in range(n_requests):
@@ -207,7 +222,12 @@ def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log
     t_plain = time.perf_counter() - t0
     res["plain"] = {"cell_updates_per_s": work / t_plain, "makespan_s": t_plain,
                     **_pcts(lat_plain)}
+    # at 70% load the served rate is the offered load; the plain path is
+    # saturated (its makespan far exceeds the arrival span), so the capacity
+    # ratio compares the saturated server with it
     res["served_over_plain"] = res["served"]["cell_updates_per_s"] / res["plain"]["cell_updates_per_s"]
+    res["served_capacity_over_plain"] = (res["served_saturated"]["cell_updates_per_s"]
+                                         / res["plain"]["cell_updates_per_s"])
     res["served_equals_plain"] = bool(same)
     return res
 

@@ -155,9 +155,44 @@ int tws_step_profiled(tws_sim* sim, int32_t n, float* launch_ms, int32_t capacit
 
 /* Frame-time driver with the reference's accumulator and 10-step clamp
  * (Terrain::PerformSimulationStep, Terrain.cpp:240-249).  step_seconds is the
- * step length; *steps_done (may be NULL) receives the number of steps run. */
+ * step length; *steps_done (may be NULL) receives the number of steps run.
+ * With a mip target set (tws_set_frame_mips), a frame that ran at least one
+ * step also regenerates the mip chain, as the reference does
+ * (Terrain.cpp:270-276). */
 int tws_perform_simulation_step(tws_sim* sim, double frame_seconds, double step_seconds,
                                 int32_t* steps_done);
+
+/* ---- terrain mip chain ---------------------------------------------------- */
+/*
+ * The full mip chain of the reference-layout TerrainData texture (r = terrain,
+ * g = b = gb_fill, a = depth), regenerated by the reference after every frame
+ * with steps (Terrain.cpp:270-276 -> Texture2D::GenMipMaps,
+ * Texture2D.cpp:64-68, glGenerateMipmap).  Level 0 is the state itself
+ * (tws_download_reference_layout); levels 1..levels-1 are stored one after
+ * the other, level l being max(1, W>>l) x max(1, H>>l) RGBA32F texels, row
+ * by row.  Each level is a bilinear reduction of the one above: bit-identical
+ * to Mesa llvmpipe's glGenerateMipmap where a dimension halves exactly (every
+ * level of a power-of-two grid); other ratios sample at the exact texel-centre
+ * position, where llvmpipe's rasteriser rounds the position by up to ~1e-4
+ * texel.
+ */
+
+/* Number of levels (floor(log2(max(W,H))) + 1, Texture.cpp:28-44) and the
+ * floats levels 1.. take together.  Either output may be NULL. */
+int tws_mip_chain_size(int32_t width, int32_t height, int32_t* levels, size_t* floats);
+
+/* Writes levels 1.. into device memory (16-byte aligned, `capacity_floats`
+ * floats).  Asynchronous on the handle's stream. */
+int tws_generate_mips(tws_sim* sim, float gb_fill, float* device_rgba, size_t capacity_floats);
+
+/* The same chain copied to host memory (the handle's idle state buffers
+ * serve as staging).  Synchronises the stream. */
+int tws_download_mips(tws_sim* sim, float gb_fill, float* host_rgba, size_t capacity_floats);
+
+/* Device buffer that tws_perform_simulation_step keeps up to date (the
+ * texture the reference's renderer samples); NULL switches it off.  The
+ * buffer stays owned by the caller and must outlive its use. */
+int tws_set_frame_mips(tws_sim* sim, float* device_rgba, size_t capacity_floats, float gb_fill);
 
 /* Total water volume sum(d) over the domain, accumulated in FP64 in a fixed
  * order (deterministic).  Synchronises the stream. */

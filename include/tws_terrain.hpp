@@ -16,6 +16,8 @@
 //     default lake depth max(0, (0.45 - 800 r^4) * 300 - b), with r the
 //     distance from the grid centre in units of the grid side
 //     (Terrain.cpp:200-238); the terrain noise is this library's seeded fBm.
+//   - after a frame with steps the TerrainData mip chain is regenerated into
+//     the target set by SetMipChainTarget (Terrain.cpp:270-276).
 // Error behaviour: the reference's methods return void and report failures
 // through its log; here they return void as well, print the library's
 // message to stderr and keep it for Ok()/LastError().
@@ -110,6 +112,22 @@ public:
         Check(tws_download_reference_layout(sim_, terrain_rgba ? terrain_rgba->data() : nullptr,
                                             0.3f, flow_rgba ? flow_rgba->data() : nullptr,
                                             flowmap_rg16 ? flowmap_rg16->data() : nullptr));
+    }
+    // TerrainData mip chain (levels 1.., Terrain.cpp:270-276).  With a device
+    // target set, PerformSimulationStep regenerates it after every frame with
+    // steps, as the reference does with glGenerateMipmap; the buffer (at
+    // least MipChainFloats() floats, 16-byte aligned) stays the caller's.
+    size_t MipChainFloats() const {
+        size_t f = 0;
+        tws_mip_chain_size((int32_t)resolution_, (int32_t)resolution_, nullptr, &f);
+        return f;
+    }
+    void SetMipChainTarget(float* device_rgba, size_t capacity_floats) {
+        Check(tws_set_frame_mips(sim_, device_rgba, capacity_floats, 0.3f));
+    }
+    void ReadMipChain(std::vector<float>* rgba) {
+        rgba->resize(MipChainFloats());
+        Check(tws_download_mips(sim_, 0.3f, rgba->data(), rgba->size()));
     }
     // Device planes for interop with device-side consumers.
     tws_device_planes DevicePlanes() const {

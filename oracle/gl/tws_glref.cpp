@@ -106,6 +106,7 @@ const __DRIextension* g_loader_exts[] = {&g_loader_ext.base, nullptr};
     X(PFNGLGETACTIVEUNIFORMSIVPROC, glGetActiveUniformsiv)                 \
     X(PFNGLDISPATCHCOMPUTEPROC, glDispatchCompute)                         \
     X(PFNGLMEMORYBARRIERPROC, glMemoryBarrier)                             \
+    X(PFNGLGENERATEMIPMAPPROC, glGenerateMipmap)                           \
     X(PFNGLFINISHPROC, glFinish)
 GL_FUNCS(GLFN)
 
@@ -389,6 +390,39 @@ int glref_run(int W, int H, const float* terrain_rgba, const float* flow_rgba, f
     GLuint texs[3] = {terrain, flow, flowmap};
     glDeleteTextures(3, texs);
     return gl_ok("readback") ? 0 : -1;
+}
+
+// The mip chain of an RGBA32F texture as the reference builds it after each
+// frame with steps: storage for `levels` levels (glTexStorage2D,
+// Texture2D.cpp:12-21; a full chain is floor(log2(max(W,H))) + 1 levels,
+// Texture.cpp:28-44), level 0 uploaded, then glGenerateMipmap
+// (Texture2D::GenMipMaps, Texture2D.cpp:64-68).  Levels 1..levels-1 are read
+// back one after the other into `out` (level l is max(1, W>>l) x
+// max(1, H>>l) RGBA texels).  *seconds: wall time of glGenerateMipmap.
+int glref_genmips(int W, int H, int levels, const float* rgba, float* out, double* seconds) {
+    if (!g_ctx) { g_err = "glref_init not called"; return -1; }
+    if (W <= 0 || H <= 0 || levels < 2) { g_err = "genmips: bad size or level count"; return -1; }
+    GLuint t = 0;
+    glGenTextures(1, &t);
+    glBindTexture(GL_TEXTURE_2D, t);
+    glTexStorage2D(GL_TEXTURE_2D, levels, GL_RGBA32F, W, H);
+    glTexSubImage2D(GL_TEXTURE_2D, 0, 0, 0, W, H, GL_RGBA, GL_FLOAT, rgba);
+    if (!gl_ok("genmips: setup")) return -1;
+    glFinish();
+    auto t0 = std::chrono::steady_clock::now();
+    glGenerateMipmap(GL_TEXTURE_2D);
+    glFinish();
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (!gl_ok("genmips: generate")) return -1;
+    size_t off = 0;
+    for (int l = 1; l < levels; ++l) {
+        const int w = W >> l > 0 ? W >> l : 1, h = H >> l > 0 ? H >> l : 1;
+        glGetTexImage(GL_TEXTURE_2D, l, GL_RGBA, GL_FLOAT, out + off);
+        off += (size_t)w * h * 4;
+    }
+    glDeleteTextures(1, &t);
+    return gl_ok("genmips: readback") ? 0 : -1;
 }
 
 }  // extern "C"

@@ -98,3 +98,89 @@ double oracle_sum_f32(const float* a, int64_t n)
     for (int64_t i = 0; i < n; ++i) s += (double)a[i];
     return s;
 }
+
+/* ---- terrain mip chain ------------------------------------------------------
+ * Terrain::PerformSimulationStep regenerates the TerrainData mip chain after
+ * every frame with steps (Terrain.cpp:270-276 -> Texture2D::GenMipMaps,
+ * Texture2D.cpp:64-68: glGenerateMipmap).  The chain is full:
+ * floor(log2(max(W,H))) + 1 levels (Texture.cpp:28-44), level l being
+ * max(1, W>>l) x max(1, H>>l).
+ *
+ * Mesa's llvmpipe builds level l+1 by a bilinear blit of level l: each
+ * texel samples level l at its own centre, u = (i+0.5)*ws/wd - 0.5, with
+ * lerp(a, b, t) = a + t*(b - a), first along x, then along y, clamped to the
+ * edge.  For halving (ws = 2*wd) this is t = 0.5 between texels 2i and 2i+1,
+ * which the llvmpipe fixtures pin bit for bit.  For any other ratio llvmpipe
+ * takes the sample position from its rasteriser's interpolation, with
+ * rounding of up to ~1e-4 texel; here the position is computed exactly (in
+ * double, t rounded once to float), so those levels agree with llvmpipe to
+ * within that rounding only. */
+static void mip_axis(int i, int ws, int wd, int* i0, int* i1, float* t)
+{
+    if (ws == 2 * wd) {
+        *i0 = 2 * i;
+        *i1 = 2 * i + 1;
+        *t = 0.5f;
+        return;
+    }
+    const double u = ((double)i + 0.5) * (double)ws / (double)wd - 0.5;
+    double fl = floor(u);
+    if (fl < 0.0) fl = 0.0;
+    int a = (int)fl;
+    if (a > ws - 1) a = ws - 1;
+    *i0 = a;
+    *i1 = a + 1 < ws ? a + 1 : ws - 1;
+    *t = (float)(u - (double)a);
+    if (*t < 0.0f) *t = 0.0f;
+}
+
+static float mip_lerp(float a, float b, float t) { return a + t * (b - a); }
+
+/* One level: src ws x hs RGBA -> dst wd x hd RGBA. */
+static void mip_level(const float* src, int ws, int hs, float* dst, int wd, int hd)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < hd; ++j) {
+        int y0, y1;
+        float ty;
+        mip_axis(j, hs, hd, &y0, &y1, &ty);
+        for (int i = 0; i < wd; ++i) {
+            int x0, x1;
+            float tx;
+            mip_axis(i, ws, wd, &x0, &x1, &tx);
+            for (int c = 0; c < 4; ++c) {
+                const float s00 = src[((size_t)y0 * ws + x0) * 4 + c];
+                const float s01 = src[((size_t)y0 * ws + x1) * 4 + c];
+                const float s10 = src[((size_t)y1 * ws + x0) * 4 + c];
+                const float s11 = src[((size_t)y1 * ws + x1) * 4 + c];
+                dst[((size_t)j * wd + i) * 4 + c] =
+                    mip_lerp(mip_lerp(s00, s01, tx), mip_lerp(s10, s11, tx), ty);
+            }
+        }
+    }
+}
+
+int oracle_mip_levels(int W, int H)
+{
+    int n = 0;
+    while (W > 0 || H > 0) { W /= 2; H /= 2; ++n; }
+    return n;
+}
+
+/* Levels 1..levels-1 of the chain whose level 0 is the RGBA texture
+ * `rgba0` (W x H), written one after the other into `out`. */
+void oracle_mips(int W, int H, const float* rgba0, float* out)
+{
+    const int levels = oracle_mip_levels(W, H);
+    const float* src = rgba0;
+    int ws = W, hs = H;
+    float* dst = out;
+    for (int l = 1; l < levels; ++l) {
+        const int wd = W >> l > 0 ? W >> l : 1, hd = H >> l > 0 ? H >> l : 1;
+        mip_level(src, ws, hs, dst, wd, hd);
+        src = dst;
+        dst += (size_t)wd * hd * 4;
+        ws = wd;
+        hs = hd;
+    }
+}

@@ -51,6 +51,9 @@ def lib() -> C.CDLL:
         h.oracle_sum_f32.restype = f64
         h.oracle_set_threads.argtypes = [i32]
         h.oracle_get_threads.restype = i32
+        h.oracle_mip_levels.argtypes = [i32, i32]
+        h.oracle_mip_levels.restype = i32
+        h.oracle_mips.argtypes = [i32, i32, vp, vp]
         _lib = h
     return _lib
 
@@ -113,3 +116,25 @@ def params_physical(dt=0.02, gravity=9.81, cell_size=1.0, pipe_area=1.0, damping
 def volume(d) -> float:
     a = np.ascontiguousarray(d, np.float32)
     return lib().oracle_sum_f32(_p(a), a.size)
+
+
+def mip_levels(W: int, H: int) -> int:
+    """Full mip chain length, Texture.cpp:28-44: floor(log2(max(W, H))) + 1."""
+    return int(lib().oracle_mip_levels(int(W), int(H)))
+
+
+def mips(rgba0):
+    """Levels 1.. of the mip chain of an H x W x 4 RGBA32F texture, as the
+    list of (h, w, 4) arrays glGenerateMipmap produces on llvmpipe
+    (Terrain.cpp:270-276, Texture2D.cpp:64-68)."""
+    a = np.ascontiguousarray(rgba0, np.float32)
+    H, W, _ = a.shape
+    levels = mip_levels(W, H)
+    sizes = [(max(1, H >> l), max(1, W >> l)) for l in range(1, levels)]
+    out = np.zeros(sum(h * w * 4 for h, w in sizes), np.float32)
+    lib().oracle_mips(W, H, _p(a), _p(out))
+    res, off = [], 0
+    for h, w in sizes:
+        res.append(out[off:off + h * w * 4].reshape(h, w, 4))
+        off += h * w * 4
+    return res

@@ -90,6 +90,13 @@ def float_to_half_rtz(a) -> np.ndarray:
     return out
 
 
+def mip_chain_size(width: int, height: int):
+    """(levels, floats of levels 1..) of the terrain mip chain (Texture.cpp:28-44)."""
+    n, f = C.c_int32(), C.c_size_t()
+    check(lib().tws_mip_chain_size(int(width), int(height), C.byref(n), C.byref(f)))
+    return n.value, f.value
+
+
 def stream_copy_gbps(nbytes: int = 1 << 30, iterations: int = 20) -> float:
     """Measured device-to-device copy bandwidth (read + write bytes / s, GB/s)."""
     g = C.c_double()
@@ -186,6 +193,28 @@ class Simulation:
         n = C.c_int32()
         check(lib().tws_perform_simulation_step(self._h, frame_seconds, step_seconds, C.byref(n)))
         return n.value
+
+    # -- terrain mip chain (Terrain.cpp:270-276)
+    def mips(self, gb_fill=0.3):
+        """Levels 1.. of the TerrainData mip chain as (h, w, 4) float32 arrays."""
+        levels, floats = mip_chain_size(self.width, self.height)
+        out = np.empty(max(floats, 1), np.float32)
+        check(lib().tws_download_mips(self._h, gb_fill, _ptr(out), floats))
+        res, off = [], 0
+        for lv in range(1, levels):
+            h, w = max(1, self.height >> lv), max(1, self.width >> lv)
+            res.append(out[off:off + h * w * 4].reshape(h, w, 4))
+            off += h * w * 4
+        return res
+
+    def generate_mips(self, device_ptr: int, capacity_floats: int, gb_fill=0.3):
+        """Writes levels 1.. into device memory at device_ptr (async)."""
+        check(lib().tws_generate_mips(self._h, gb_fill, C.c_void_p(device_ptr), capacity_floats))
+
+    def set_frame_mips(self, device_ptr: int | None, capacity_floats: int = 0, gb_fill=0.3):
+        """Device buffer perform_simulation_step keeps current (None: off)."""
+        check(lib().tws_set_frame_mips(self._h, C.c_void_p(device_ptr) if device_ptr else None,
+                                       capacity_floats, gb_fill))
 
     def set_launch(self, kernel: int, steps_per_launch: int):
         check(lib().tws_set_launch(self._h, kernel, steps_per_launch))

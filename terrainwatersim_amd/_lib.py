@@ -139,6 +139,10 @@ PROTOTYPES = {
     "tws_step": (C.c_int, [_vp, _i32]),
     "tws_step_profiled": (C.c_int, [_vp, _i32, _fp, _i32, C.POINTER(_i32)]),
     "tws_perform_simulation_step": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(_i32)]),
+    "tws_mip_chain_size": (C.c_int, [_i32, _i32, C.POINTER(_i32), C.POINTER(C.c_size_t)]),
+    "tws_generate_mips": (C.c_int, [_vp, C.c_float, _vp, C.c_size_t]),
+    "tws_download_mips": (C.c_int, [_vp, C.c_float, _vp, C.c_size_t]),
+    "tws_set_frame_mips": (C.c_int, [_vp, _vp, C.c_size_t, C.c_float]),
     "tws_volume": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "tws_check_finite": (C.c_int, [_vp]),
     "tws_sync": (C.c_int, [_vp]),

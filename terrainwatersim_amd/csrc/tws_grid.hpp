@@ -60,6 +60,11 @@ int grid_init_terrain_fbm(GridState& g, uint64_t seed, int octaves, int base, fl
 int grid_init_water(GridState& g, float depth, int x0, int y0, int x1, int y1, float rect_depth,
                     hipStream_t s);
 
+// Terrain mip chain (tws_mips.hip): level count and floats of levels 1..,
+// and the chain written into device memory `dst` on stream s.
+int mip_chain(int W, int H, int* levels, size_t* floats);
+int grid_mips(GridState& g, float gb_fill, float* dst, size_t capacity_floats, hipStream_t s);
+
 // Tiles of a grid for a (kernel variant, steps per launch) pair.
 bool grid_tiles(int W, int H, int kernel, int k, int* tiles_x, int* tiles_y);
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <new>
 #include <vector>
 
@@ -21,6 +22,9 @@ struct tws_sim {
     int kernel = TWS_KERNEL_TILE_128x64;
     int k = 4;
     double time_acc = 0.0;       // PerformSimulationStep accumulator
+    float* frame_mips = nullptr; // mip chain kept current per frame (caller's buffer)
+    size_t frame_mips_cap = 0;
+    float frame_mips_gb = 0.3f;
 };
 
 #define TWS_HIP(call, what)                              \
@@ -217,7 +221,59 @@ int tws_perform_simulation_step(tws_sim* s, double frame_seconds, double step_se
     s->time_acc -= step_seconds * n;
     if (n > 10) n = 10;
     if (steps_done) *steps_done = (int32_t)n;
-    return tws_step(s, (int32_t)n);
+    const int rc = tws_step(s, (int32_t)n);
+    if (rc != TWS_OK || n == 0 || !s->frame_mips) return rc;
+    // Terrain.cpp:270-276: the mip chain follows every frame with steps
+    return grid_mips(s->g, s->frame_mips_gb, s->frame_mips, s->frame_mips_cap, s->stream);
+}
+
+int tws_mip_chain_size(int32_t width, int32_t height, int32_t* levels, size_t* floats) {
+    if (width <= 0 || height <= 0)
+        return fail(TWS_ERR_INVALID, "tws_mip_chain_size: invalid size %dx%d", width, height);
+    int n = 0;
+    mip_chain(width, height, &n, floats);
+    if (levels) *levels = n;
+    return TWS_OK;
+}
+
+int tws_generate_mips(tws_sim* s, float gb_fill, float* device_rgba, size_t capacity_floats) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_generate_mips: NULL handle");
+    return grid_mips(s->g, gb_fill, device_rgba, capacity_floats, s->stream);
+}
+
+int tws_download_mips(tws_sim* s, float gb_fill, float* host_rgba, size_t capacity_floats) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_download_mips: NULL handle");
+    size_t need = 0;
+    mip_chain(s->g.W, s->g.H, nullptr, &need);
+    if (need == 0) return TWS_OK;
+    if (!host_rgba || capacity_floats < need)
+        return fail(TWS_ERR_INVALID, "tws_download_mips: buffer holds %zu floats, the chain needs %zu",
+                    host_rgba ? capacity_floats : (size_t)0, need);
+    // the idle flux set: 4 * pitch * H floats, more than any chain of the grid
+    float* st = s->g.staging();
+    const int rc = grid_mips(s->g, gb_fill, st, 4 * (size_t)s->g.pitch * s->g.H, s->stream);
+    if (rc != TWS_OK) return rc;
+    TWS_HIP(hipMemcpyAsync(host_rgba, st, sizeof(float) * need, hipMemcpyDeviceToHost, s->stream),
+            "tws_download_mips: copy");
+    TWS_HIP(hipStreamSynchronize(s->stream), "tws_download_mips: sync");
+    return TWS_OK;
+}
+
+int tws_set_frame_mips(tws_sim* s, float* device_rgba, size_t capacity_floats, float gb_fill) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_set_frame_mips: NULL handle");
+    if (device_rgba) {
+        size_t need = 0;
+        mip_chain(s->g.W, s->g.H, nullptr, &need);
+        if (capacity_floats < need)
+            return fail(TWS_ERR_INVALID, "tws_set_frame_mips: buffer holds %zu floats, the chain needs %zu",
+                        capacity_floats, need);
+        if (reinterpret_cast<uintptr_t>(device_rgba) % 16 != 0 || !std::isfinite(gb_fill))
+            return fail(TWS_ERR_INVALID, "tws_set_frame_mips: unaligned buffer or non-finite fill");
+    }
+    s->frame_mips = device_rgba;
+    s->frame_mips_cap = device_rgba ? capacity_floats : 0;
+    s->frame_mips_gb = gb_fill;
+    return TWS_OK;
 }
 
 int tws_volume(tws_sim* s, double* out) {

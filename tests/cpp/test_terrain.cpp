@@ -81,6 +81,28 @@ int main() {
             if (f1[i * 4 + c] != ff[i * 4 + c]) ++diff;
     }
     EXPECT(diff == 0);
+
+    // mip chain of the current TerrainData: 9 levels for 256^2 (Texture.cpp:28-44),
+    // level 1 = 128^2 texels of (r, 0.3, 0.3, a), each the half-weight bilinear
+    // blend of its 2x2 block of level 0
+    std::vector<float> mips;
+    terrain.ReadMipChain(&mips);
+    size_t want = 0;
+    for (int l = 1; l < 9; ++l) want += 4u * (N >> l) * (N >> l);
+    EXPECT(terrain.MipChainFloats() == want && mips.size() == want);
+    int mip_bad = 0;
+    auto lerp = [](float a, float b2) { return a + 0.5f * (b2 - a); };
+    for (int j = 0; j < N / 2; j += 17)
+        for (int i = 0; i < N / 2; i += 13)
+            for (int c = 0; c < 4; ++c) {
+                const float* q = t1.data();
+                const float s00 = q[((2 * j) * N + 2 * i) * 4 + c], s01 = q[((2 * j) * N + 2 * i + 1) * 4 + c];
+                const float s10 = q[((2 * j + 1) * N + 2 * i) * 4 + c];
+                const float s11 = q[((2 * j + 1) * N + 2 * i + 1) * 4 + c];
+                if (mips[((size_t)j * (N / 2) + i) * 4 + c] != lerp(lerp(s00, s01), lerp(s10, s11)))
+                    ++mip_bad;
+            }
+    EXPECT(mip_bad == 0);
     EXPECT(terrain.Ok());
     std::printf(failures ? "FAILED %d\n" : "OK\n", failures);
     return failures ? 1 : 0;

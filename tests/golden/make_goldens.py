@@ -16,7 +16,13 @@ unpadded, so the shaders read texels outside the image.
 Also records the llvmpipe timing of the 256^2 rain workload (1000 steps) and
 of 20 steps of a 4096^2 grid as profiles/cpu_baseline_llvmpipe.json.
 
-    python tests/golden/make_goldens.py [--shaders DIR] [--skip-timing]
+Mip fixtures (mips_*.npz): the TerrainData mip chain that
+Terrain::PerformSimulationStep regenerates after each frame with steps
+(Terrain.cpp:270-276 -> Texture2D::GenMipMaps, Texture2D.cpp:64-68), i.e.
+glGenerateMipmap on a full-chain RGBA32F texture (r = b, g = b = 0.3,
+a = d), levels 1.. read back from llvmpipe.
+
+    python tests/golden/make_goldens.py [--shaders DIR] [--skip-timing] [--mips-only]
 """
 from __future__ import annotations
 
@@ -82,6 +88,29 @@ class GLRef:
             raise RuntimeError(self.lib.glref_last_error().decode())
         return ot[..., 3].copy(), of, om, sec.value
 
+    def mips(self, b, d, gb=0.3):
+        """Levels 1.. of glGenerateMipmap over the RGBA32F texture (b, gb, gb, d)."""
+        H, W = b.shape
+        t = np.zeros((H, W, 4), F32)
+        t[..., 0], t[..., 1], t[..., 2], t[..., 3] = b, gb, gb, d
+        levels = 0
+        w, h = W, H
+        while w > 0 or h > 0:
+            w, h, levels = w // 2, h // 2, levels + 1
+        sizes = [(max(1, H >> l), max(1, W >> l)) for l in range(1, levels)]
+        out = np.zeros(sum(hh * ww * 4 for hh, ww in sizes), F32)
+        sec = C.c_double()
+        fn = self.lib.glref_genmips
+        fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+        if fn(W, H, levels, t.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+              C.byref(sec)) != 0:
+            raise RuntimeError(self.lib.glref_last_error().decode())
+        res, off = [], 0
+        for hh, ww in sizes:
+            res.append(out[off:off + hh * ww * 4].reshape(hh, ww, 4).copy())
+            off += hh * ww * 4
+        return res
+
 
 def framed(b, d, f):
     H, W = b.shape
@@ -116,11 +145,41 @@ def run_case(gl, b, d, f, FKC, checkpoints, boundary, rain=None):
     return out
 
 
+def write_mips(gl, tws):
+    """Mip-chain fixtures: power-of-two shapes (llvmpipe's blit halves every
+    level exactly) and non-power-of-two ones (its sample positions carry the
+    rasteriser's rounding)."""
+    rng = np.random.default_rng(7)
+    cases = {
+        "mips_fbm128": (tws.generate_fbm_terrain(128, 128, seed=7),
+                        (rng.random((128, 128)) * 3.0 * (rng.random((128, 128)) < 0.6)).astype(F32)),
+        "mips_rect64x16": ((rng.standard_normal((16, 64)) * 50).astype(F32),
+                           (rng.random((16, 64)) * 2).astype(F32)),
+        "mips_column1x8": ((rng.standard_normal((8, 1)) * 50).astype(F32),
+                           (rng.random((8, 1)) * 2).astype(F32)),
+        "mips_npot100x60": (tws.generate_fbm_terrain(100, 60, seed=8),
+                            (rng.random((60, 100)) * 3.0).astype(F32)),
+    }
+    entries = {}
+    for name, (b, d) in cases.items():
+        levels = gl.mips(b, d)
+        arrays = {"b": b, "d": d}
+        arrays.update({f"level{l}": a for l, a in enumerate(levels, 1)})
+        np.savez_compressed(HERE / f"{name}.npz", **arrays)
+        entries[name] = {"shape": list(b.shape), "gb_fill": 0.3, "levels": len(levels) + 1,
+                         "arrays": {k: hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest()
+                                    for k, v in arrays.items()}}
+        print(f"{name}: {len(levels)} levels", flush=True)
+    return entries
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--shaders", default=os.environ.get(
         "TWS_REFERENCE_SHADERS", "/root/reference/terrainwatersim/shader"))
     ap.add_argument("--skip-timing", action="store_true")
+    ap.add_argument("--mips-only", action="store_true",
+                    help="only (re)write the mip fixtures and their manifest entries")
     args = ap.parse_args()
 
     import terrainwatersim_amd as tws   # host-only functions: params, terrain generator
@@ -139,6 +198,12 @@ def main():
         "closed_walls": f"{FRAME}-cell frame of dry terrain {WALL:g} m high around the domain",
         "cases": {},
     }
+
+    if args.mips_only:
+        old = json.loads((HERE / "manifest.json").read_text())
+        old["mip_cases"] = write_mips(gl, tws)
+        (HERE / "manifest.json").write_text(json.dumps(old, indent=1) + "\n")
+        return
 
     def rnd(seed, H, W, wet=0.7):
         rng = np.random.default_rng(seed)
@@ -186,6 +251,7 @@ def main():
                        for k, v in arrays.items()},
         }
         print(f"{name}: {time.time() - t0:.1f}s", flush=True)
+    manifest["mip_cases"] = write_mips(gl, tws)
     (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1) + "\n")
 
     if args.skip_timing:

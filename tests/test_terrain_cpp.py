@@ -13,7 +13,7 @@ def build_cpp_tests():
     """Compiles the C++ test drivers against include/ and libtws.so."""
     out = EXE.parent
     out.mkdir(exist_ok=True)
-    subprocess.run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}",
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}",
                     str(ROOT / "tests" / "cpp" / "test_terrain.cpp"), "-o", str(EXE),
                     f"-L{ROOT / 'terrainwatersim_amd'}", "-ltws",
                     "-Wl,-rpath,$ORIGIN/../../../terrainwatersim_amd"], check=True)

@@ -75,9 +75,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB STREAM copy kernel (reference for PMC byte counts)")
-    ap.add_argument("--extra", default="batched,served",
-                    help="comma list of extra workloads in the same line: batched, served, "
-                         "large ('' for none)")
+    ap.add_argument("--extra", default="small,batched,served,large",
+                    help="comma list of extra workloads in the same line: small, batched, "
+                         "served, large ('' for none)")
     ap.add_argument("--served-requests", type=int, default=200)
     args = ap.parse_args()
 
@@ -179,6 +179,8 @@ def main():
     extra = [w for w in args.extra.split(",") if w]
     if extra:
         from terrainwatersim_amd import workloads
+        if "small" in extra:
+            rec["small"] = workloads.small(log=log)
         if "batched" in extra:
             rec["batched"] = workloads.batched(log=log)
         if "served" in extra:

@@ -72,17 +72,20 @@ __device__ __forceinline__ void vload<4>(const gfloat* p, float (&o)[4]) {
     const vf4 v = *reinterpret_cast<const gfloat4*>(p);
     o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
 }
+// Stores of the new state are marked non-temporal: no tile of the same
+// launch reads them back, so they need not displace the halos other tiles
+// are about to re-read from L2 (measured +1.5-3% at 4096^2).
 template <int N>
 __device__ __forceinline__ void vstore(gfloat* p, const float (&o)[N]);
 template <>
 __device__ __forceinline__ void vstore<1>(gfloat* p, const float (&o)[1]) { p[0] = o[0]; }
 template <>
 __device__ __forceinline__ void vstore<2>(gfloat* p, const float (&o)[2]) {
-    *reinterpret_cast<gfloat2*>(p) = vf2{o[0], o[1]};
+    __builtin_nontemporal_store(vf2{o[0], o[1]}, reinterpret_cast<gfloat2*>(p));
 }
 template <>
 __device__ __forceinline__ void vstore<4>(gfloat* p, const float (&o)[4]) {
-    *reinterpret_cast<gfloat4*>(p) = vf4{o[0], o[1], o[2], o[3]};
+    __builtin_nontemporal_store(vf4{o[0], o[1], o[2], o[3]}, reinterpret_cast<gfloat4*>(p));
 }
 
 // Non-short-circuit finiteness test (no branches).

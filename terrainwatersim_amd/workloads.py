@@ -1,5 +1,8 @@
 """Benchmark workloads beside the headline plain-grid step (used by bench.py).
 
+small    256^2 fBm terrain (seed 1), dry start, rain disc r = 16 (R^2 = 256)
+         adding 1e-3 m/step at the centre before every step, closed walls,
+         1000 steps: the latency/launch-bound case (state fits in L2/MALL).
 batched  256 independent terrains, sides log-uniform 128..2048 (seed 3),
          per-grid random rain discs (r in [8, 32], 1e-3 m/step), 100 steps per
          captured launch sequence; (variant, k) from the autotuner.  Compared
@@ -39,6 +42,29 @@ def _splitmix(x: int) -> int:
 
 
 # ---------------------------------------------------------------------------
+def small(steps=1000, repeats=5, log=print):
+    p = tws.params_physical(**PARAMS)
+    N = 256
+    b = tws.generate_fbm_terrain(N, N, seed=1)
+    log("small: 256^2 rain, 1000 steps")
+    times = []
+    with tws.Simulation(N, N, p) as sim:
+        kernel, k = sim.get_launch()
+        for _ in range(repeats + 1):
+            sim.upload(b, np.zeros((N, N), F32))
+            sim.set_rain(128.0, 128.0, 1e-3, 256.0)
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(steps)
+            sim.sync()
+            times.append(time.perf_counter() - t0)
+        vol = sim.volume()
+    t = float(np.median(times[1:]))
+    return {"grid": N, "steps": steps, "kernel": kernel, "steps_per_launch": k,
+            "cell_updates_per_s": N * N * steps / t, "ms_per_1000_steps": t * 1e3 * 1000 / steps,
+            "volume_m3": vol}
+
+
 def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
     rng = np.random.default_rng(seed)
     sides = np.round(np.exp(rng.uniform(math.log(128), math.log(2048), n_grids))).astype(int)

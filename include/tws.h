@@ -216,7 +216,10 @@ int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 #define TWS_KERNEL_TILE_128x128 2   /* 1024 threads, 2 columns x 8 rows each */
 #define TWS_KERNEL_TILE_128x64 3    /*  512 threads                          */
 #define TWS_KERNEL_TILE_256x64 4    /* 1024 threads, 4 columns x 4 rows each */
-#define TWS_KERNEL_COUNT 5
+#define TWS_KERNEL_TILE_64x64 5     /*  512 threads, 1 column x 8 rows each:
+                                       more, smaller tiles for small grids,
+                                       whose steps are bound by latency     */
+#define TWS_KERNEL_COUNT 6
 
 /* Selects a kernel variant and the number of steps per launch (1, 2, 3, 4,
  * 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */

@@ -29,7 +29,8 @@ KERNEL_TWOPASS = 1
 KERNEL_TILE_128x128 = 2
 KERNEL_TILE_128x64 = 3
 KERNEL_TILE_256x64 = 4
-TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64)
+KERNEL_TILE_64x64 = 5
+TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64, KERNEL_TILE_64x64)
 
 
 class TwsError(RuntimeError):

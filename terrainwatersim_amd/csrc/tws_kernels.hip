@@ -652,6 +652,7 @@ bool tile_shape(int kernel, int k, TileShape* out) {
         case TWS_KERNEL_TILE_128x128: shape_of<2, 8, 16>(k, out); return true;
         case TWS_KERNEL_TILE_128x64: shape_of<2, 8, 8>(k, out); return true;
         case TWS_KERNEL_TILE_256x64: shape_of<4, 4, 16>(k, out); return true;
+        case TWS_KERNEL_TILE_64x64: shape_of<1, 8, 8>(k, out); return true;
         default: return false;
     }
 }
@@ -664,6 +665,7 @@ static hipError_t launch_tile_any(int kernel, int k, const GridDesc* grids_dev, 
         case TWS_KERNEL_TILE_128x128: return launch_tile_k<2, 8, 16>(k, grids_dev, hdr, single, blocks, launch_idx, s);
         case TWS_KERNEL_TILE_128x64: return launch_tile_k<2, 8, 8>(k, grids_dev, hdr, single, blocks, launch_idx, s);
         case TWS_KERNEL_TILE_256x64: return launch_tile_k<4, 4, 16>(k, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_64x64: return launch_tile_k<1, 8, 8>(k, grids_dev, hdr, single, blocks, launch_idx, s);
         default: return hipErrorInvalidValue;
     }
 }

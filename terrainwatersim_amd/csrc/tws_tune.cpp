@@ -2,9 +2,11 @@
 // table and the measuring autotuner.
 //
 // choose_launch() maps a grid size to (kernel variant, steps per launch).
-// The built-in entry comes from the MI355X sweep (128 x 64 tiles, two work
-// groups per CU so one loads while the other computes, 4 steps per launch:
-// 3.2e11 cell-updates/s at 4096^2).  tws_autotune() times every candidate on
+// The built-in table comes from an MI355X sweep over sizes 128^2..4096^2:
+// 64 x 64 tiles up to 2048^2 (small grids are bound by the latency of one
+// tile's levels, so more and smaller tiles finish sooner), 128 x 64 tiles
+// beyond (two work groups per CU so one loads while the other computes,
+// 4 steps per launch: 3.3e11 cell-updates/s at 4096^2).  tws_autotune() times every candidate on
 // scratch grids of the requested shapes and records the winner for that
 // size class (log2 of the cell count), which choose_launch() then prefers.
 
@@ -41,8 +43,21 @@ void choose_launch(int width, int height, int* kernel, int* k) {
             return;
         }
     }
-    *kernel = TWS_KERNEL_TILE_128x64;
-    *k = cells <= 256 * 256 ? 2 : 4;
+    // MI355X sweep (cell-updates/s; tiles 64x64 vs 128x64 at the best k):
+    //   <= 512^2   latency-bound; 64x64 tiles spread the grid over more CUs:
+    //              256^2 2.5e10 (64x64, k = 8) vs 1.6e10 (128x64, k = 8)
+    //   <= 2048^2  2048^2 2.6e11 (64x64, k = 4) vs 2.4e11 (128x64, k = 6)
+    //   larger     4096^2 3.2e11 (128x64, k = 4) vs 3.0e11 (64x64, k = 4)
+    if (cells <= 512 * 512) {
+        *kernel = TWS_KERNEL_TILE_64x64;
+        *k = 8;
+    } else if (cells <= 2048 * 2048) {
+        *kernel = TWS_KERNEL_TILE_64x64;
+        *k = 4;
+    } else {
+        *kernel = TWS_KERNEL_TILE_128x64;
+        *k = 4;
+    }
 }
 
 }  // namespace tws
@@ -83,7 +98,7 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
         if (rc == TWS_OK) rc = tws_batch_upload(b, id, terrain.data(), depth.data(), nullptr);
     }
     static const int kernels[] = {TWS_KERNEL_TILE_128x64, TWS_KERNEL_TILE_128x128,
-                                  TWS_KERNEL_TILE_256x64};
+                                  TWS_KERNEL_TILE_256x64, TWS_KERNEL_TILE_64x64};
     static const int ks[] = {2, 3, 4, 6, 8};
     tws_tune_entry top{0, 0, 0.0};
     int m = 0;

@@ -55,7 +55,8 @@ def test_lane_shift_layout(gpu):
     assert out[64:].tolist() == list(range(2, 65)) + [0]
 
 
-KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 8), (4, 1), (4, 2), (4, 4), (4, 8)]
+KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 8), (4, 1), (4, 2), (4, 4), (4, 8),
+           (5, 1), (5, 4), (5, 6), (5, 8)]
 
 
 @pytest.mark.parametrize("kernel,k", KERNELS)

@@ -127,52 +127,10 @@ __device__ __forceinline__ void flux_update(float F, float K, float C, float d, 
 }
 
 // ---------------------------------------------------------------------------
-// Fused, temporally blocked step.  Grid: one block per tile of the flattened
-// tile list.  Single grid: `grids` is null, g1 (by value) is the grid and
-// the tile count is gridDim.x.  Batch: hdr = {number of grids, number of
-// tiles} and grids[] (ordered by tile_begin) live in device memory, so a
-// captured launch stays valid while grids come and go; blocks past the tile
-// count exit at once.
-template <int CPL, int RPW, int NW, int KH>
-__global__ void __launch_bounds__(NW * kWave)
-k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ hdr,
-       int launch_idx) {
-    using G = TileGeom<CPL, RPW, NW, KH>;
-    constexpr int EXW = G::EXW, EXH = G::EXH;
-    __shared__ float s_b[EXH * EXW];
-    __shared__ float s_xh[2][NW][CPL][kWave];  // [0]: first row of a wave, [1]: last row
-    __shared__ float s_xf[2][NW][CPL][kWave];
-
-    // XCD-aware order: the hardware deals blocks to the 8 XCDs round robin;
-    // give each XCD a contiguous run of tiles so neighbouring tiles (whose
-    // halos overlap) share one L2.
-    const int ngrids = grids ? hdr[0] : 0;
-    const int total_tiles = grids ? hdr[1] : (int)gridDim.x;
-    const int bid = blockIdx.x;
-    if (bid >= total_tiles) return;
-    const int xcd = bid & 7, idx = bid >> 3;
-    const int q = total_tiles >> 3, rmd = total_tiles & 7;
-    const int tile = xcd * q + (xcd < rmd ? xcd : rmd) + idx;
-
-    GridDesc g;
-    if (!grids) {
-        g = g1;
-    } else {
-        int lo = 0, hi = ngrids - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (grids[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
-        }
-        g = grids[lo];
-    }
-    const int levels = min(KH, g.steps - launch_idx * KH);
-    if (levels <= 0) return;
-    const int parity = (g.parity0 + launch_idx) & 1;
-    const int lt = tile - g.tile_begin;
-    // Tile order inside the grid: groups of tile_group columns, row by row
-    // within a group, so the tiles in flight at once form a compact block
-    // whose shared halos meet in L2 (tile_group 0: plain row-major).
-    int tx, ty;
+// Tile (tx, ty) of a grid's tile list position lt: groups of tile_group tile
+// columns, row by row within a group, so the tiles in flight at once form a
+// compact block whose shared halos meet in L2 (tile_group 0: row-major).
+__device__ __forceinline__ void tile_coords(const GridDesc& g, int lt, int& tx, int& ty) {
     if (g.tile_group > 0) {
         const int per_group = g.tile_group * g.tiles_y;
         const int grp = lt / per_group, in_grp = lt - grp * per_group;
@@ -184,6 +142,20 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ 
         ty = lt / g.tiles_x;
         tx = lt - ty * g.tiles_x;
     }
+}
+
+// One tile: load its EXW x EXH block of state set `parity`, advance it
+// `levels` steps, store the valid interior into the other set.  Returns
+// whether this lane stored a non-finite value.
+template <int CPL, int RPW, int NW, int KH>
+__device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int levels,
+                                          int parity) {
+    using G = TileGeom<CPL, RPW, NW, KH>;
+    constexpr int EXW = G::EXW, EXH = G::EXH;
+    __shared__ float s_b[EXH * EXW];
+    __shared__ float s_xh[2][NW][CPL][kWave];  // [0]: first row of a wave, [1]: last row
+    __shared__ float s_xf[2][NW][CPL][kWave];
+
     const int ox = tx * G::OX - G::HALOX;
     const int oy = ty * G::OY - G::HALOY;
     const int W = g.W, H = g.H;
@@ -372,7 +344,49 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ 
             }
         }
     }
-    if (__ballot(bad) != 0 && lane == 0) atomicOr(g.status, 1u);
+    return bad;
+}
+
+// ---------------------------------------------------------------------------
+// Fused, temporally blocked step.  Grid: one block per tile of the flattened
+// tile list.  Single grid: `grids` is null, g1 (by value) is the grid and
+// the tile count is gridDim.x.  Batch: hdr = {number of grids, number of
+// tiles} and grids[] (ordered by tile_begin) live in device memory, so a
+// captured launch stays valid while grids come and go; blocks past the tile
+// count exit at once.
+template <int CPL, int RPW, int NW, int KH>
+__global__ void __launch_bounds__(NW * kWave)
+k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ hdr,
+       int launch_idx) {
+    // XCD-aware order: the hardware deals blocks to the 8 XCDs round robin;
+    // give each XCD a contiguous run of tiles so neighbouring tiles (whose
+    // halos overlap) share one L2.
+    const int ngrids = grids ? hdr[0] : 0;
+    const int total_tiles = grids ? hdr[1] : (int)gridDim.x;
+    const int bid = blockIdx.x;
+    if (bid >= total_tiles) return;
+    const int xcd = bid & 7, idx = bid >> 3;
+    const int q = total_tiles >> 3, rmd = total_tiles & 7;
+    const int tile = xcd * q + (xcd < rmd ? xcd : rmd) + idx;
+
+    GridDesc g;
+    if (!grids) {
+        g = g1;
+    } else {
+        int lo = 0, hi = ngrids - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (grids[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+        }
+        g = grids[lo];
+    }
+    const int levels = min(KH, g.steps - launch_idx * KH);
+    if (levels <= 0) return;
+    const int parity = (g.parity0 + launch_idx) & 1;
+    int tx, ty;
+    tile_coords(g, tile - g.tile_begin, tx, ty);
+    const bool bad = tile_step<CPL, RPW, NW, KH>(g, tx, ty, levels, parity);
+    if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(g.status, 1u);
 }
 
 // ---------------------------------------------------------------------------

@@ -398,6 +398,11 @@ int tws_server_wait(tws_server* server, uint64_t ticket, double timeout_seconds,
 int tws_server_cancel(tws_server* server, uint64_t ticket);
 /* Blocks until every submitted request has finished. */
 int tws_server_drain(tws_server* server);
+/* Forgets a finished request (DONE, FAILED or CANCELLED): its ticket is no
+ * longer valid and the server frees its record.  TWS_ERR_STATE while it is
+ * queued or running.  A long-lived server's callers release every ticket
+ * they have consumed. */
+int tws_server_release(tws_server* server, uint64_t ticket);
 int tws_server_stats_get(tws_server* server, tws_server_stats* out);
 /* Server monotonic clock, in the time base of tws_request_status. */
 double tws_server_now(const tws_server* server);

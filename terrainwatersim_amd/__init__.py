@@ -455,7 +455,9 @@ class Server:
         return ticket.value, out
 
     def release(self, ticket):
-        """Forgets the output arrays of a finished request."""
+        """Forgets a finished request: the server's record and the output
+        arrays kept for it (tws_server_release)."""
+        check(lib().tws_server_release(self._h, ticket))
         self._keep.pop(ticket, None)
 
     def poll(self, ticket) -> RequestStatus:

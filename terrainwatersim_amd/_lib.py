@@ -186,6 +186,7 @@ PROTOTYPES = {
     "tws_server_wait": (C.c_int, [_vp, C.c_uint64, C.c_double, C.POINTER(RequestStatus)]),
     "tws_server_cancel": (C.c_int, [_vp, C.c_uint64]),
     "tws_server_drain": (C.c_int, [_vp]),
+    "tws_server_release": (C.c_int, [_vp, C.c_uint64]),
     "tws_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "tws_server_now": (C.c_double, [_vp]),
     "tws_server_destroy": (C.c_int, [_vp]),

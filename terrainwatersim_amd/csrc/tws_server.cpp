@@ -18,7 +18,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -218,6 +220,10 @@ void tws_server::loop() {
                     int gid = -1;
                     if (tws_batch_add(batch, q->r.width, q->r.height, &q->r.params, &gid) == TWS_OK) {
                         q->gid = gid;
+                        // RUNNING from here on: a cancel now only flags the request,
+                        // which the loop retires once its upload is done
+                        q->st.state = TWS_REQ_RUNNING;
+                        q->st.t_admit = now();
                         admit.push_back(q);
                         it = queue.erase(it);
                         --slots;
@@ -283,8 +289,6 @@ void tws_server::loop() {
                     continue;
                 }
                 free_inputs(q);
-                q->st.state = TWS_REQ_RUNNING;
-                q->st.t_admit = now();
                 resident.push_back(q);
             }
         }
@@ -452,6 +456,19 @@ int tws_server_stats_get(tws_server* s, tws_server_stats* out) {
     if (!s || !out) return fail(TWS_ERR_INVALID, "tws_server_stats_get: NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->stats;  // pool_used as of the last batched run
+    return TWS_OK;
+}
+
+int tws_server_release(tws_server* s, uint64_t ticket) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_server_release: NULL handle");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return fail(TWS_ERR_INVALID, "tws_server_release: unknown ticket");
+    const int st = it->second->st.state;
+    if (st == TWS_REQ_QUEUED || st == TWS_REQ_RUNNING)
+        return fail(TWS_ERR_STATE, "tws_server_release: request %llu not finished",
+                    (unsigned long long)ticket);
+    s->all.erase(it);
     return TWS_OK;
 }
 

@@ -23,7 +23,6 @@ from __future__ import annotations
 
 import hashlib
 import math
-import threading
 import time
 
 import numpy as np

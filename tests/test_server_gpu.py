@@ -106,3 +106,23 @@ def test_cancel_and_nonfinite_are_isolated(gpu):
         assert st_bad.state == gpu.REQ_FAILED and st_bad.error == -4
         assert srv.poll(t_cancel).state == gpu.REQ_CANCELLED
         assert np.array_equal(out_ok["depth"], alone(gpu, b, d, 60, p)["depth"])
+
+
+def test_release_forgets_finished_requests_only(gpu):
+    p = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995)
+    with gpu.Server(GB, max_resident=4, chunk_steps=10) as srv:
+        b, d = request(gpu, 11, 128, 128)
+        t_long, _ = srv.submit(b, 5000, p, depth=d)
+        t_short, out = srv.submit(b, 3, p, depth=d)
+        st = srv.wait(t_short, 60.0)
+        assert st.state == gpu.REQ_DONE
+        with pytest.raises(gpu.TwsError):   # still running: its record stays
+            srv.release(t_long)
+        srv.release(t_short)
+        with pytest.raises(gpu.TwsError):   # forgotten
+            srv.poll(t_short)
+        srv.cancel(t_long)
+        srv.drain()
+        assert srv.poll(t_long).state == gpu.REQ_CANCELLED
+        srv.release(t_long)
+        assert srv.stats()["completed"] == 1 and srv.stats()["cancelled"] == 1

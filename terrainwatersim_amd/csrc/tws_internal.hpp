@@ -75,9 +75,12 @@ bool tile_shape(int kernel, int k, TileShape* out);
 
 // Launchers (stream-ordered, return hipError_t).
 // Batch launch: hdr_dev = {number of grids, number of tiles}; `blocks` is an
-// upper bound of the tile count (extra blocks exit).
-hipError_t launch_tile_batch(int kernel, int k, const GridDesc* grids_dev, const int* hdr_dev,
-                             int blocks, int launch_idx, hipStream_t s);
+// upper bound of the tile count (extra blocks exit).  `general` must be true
+// if any grid of the table has rain or an edge other than closed walls
+// (needs_general_kernel); the other kernel compiles those paths out.
+bool needs_general_kernel(const GridDesc& g);
+hipError_t launch_tile_batch(int kernel, int k, bool general, const GridDesc* grids_dev,
+                             const int* hdr_dev, int blocks, int launch_idx, hipStream_t s);
 hipError_t launch_twopass(const GridDesc* g_dev, const GridDesc& g_host, int parity,
                           hipStream_t s);
 hipError_t launch_brush(const GridDesc& g, int parity, float cx, float cy, float I, float R2,

@@ -222,7 +222,7 @@ int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 #define TWS_KERNEL_COUNT 6
 
 /* Selects a kernel variant and the number of steps per launch (1, 2, 3, 4,
- * 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */
+ * 5, 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */
 int tws_set_launch(tws_sim* sim, int32_t kernel, int32_t steps_per_launch);
 int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launch);
 /* Order in which the step kernel walks a grid's tiles: groups of

@@ -640,6 +640,7 @@ static hipError_t launch_tile_kg(int k, const GridDesc* grids_dev, const int* hd
         case 2: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 2, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
         case 3: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 3, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
         case 4: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 4, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 5: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 5, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
         case 6: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 6, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
         case 8: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 8, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
         default: return hipErrorInvalidValue;
@@ -665,6 +666,7 @@ static void shape_of(int k, TileShape* out) {
         case 2: ox = TileGeom<CPL, RPW, NW, 2>::OX; oy = TileGeom<CPL, RPW, NW, 2>::OY; break;
         case 4: ox = TileGeom<CPL, RPW, NW, 4>::OX; oy = TileGeom<CPL, RPW, NW, 4>::OY; break;
         case 3: ox = TileGeom<CPL, RPW, NW, 3>::OX; oy = TileGeom<CPL, RPW, NW, 3>::OY; break;
+        case 5: ox = TileGeom<CPL, RPW, NW, 5>::OX; oy = TileGeom<CPL, RPW, NW, 5>::OY; break;
         case 6: ox = TileGeom<CPL, RPW, NW, 6>::OX; oy = TileGeom<CPL, RPW, NW, 6>::OY; break;
         case 8: ox = TileGeom<CPL, RPW, NW, 8>::OX; oy = TileGeom<CPL, RPW, NW, 8>::OY; break;
     }
@@ -672,7 +674,7 @@ static void shape_of(int k, TileShape* out) {
 }
 
 bool tile_shape(int kernel, int k, TileShape* out) {
-    if (k != 1 && k != 2 && k != 3 && k != 4 && k != 6 && k != 8) return false;
+    if (k < 1 || k > 8 || k == 7) return false;
     switch (kernel) {
         case TWS_KERNEL_TILE_128x128: shape_of<2, 8, 16>(k, out); return true;
         case TWS_KERNEL_TILE_128x64: shape_of<2, 8, 8>(k, out); return true;

@@ -62,7 +62,7 @@ int step_grid(GridState& gs, int kernel, int k, int32_t n, hipStream_t stream, h
     while (left > 0) {
         int kk = k;
         if (left < kk) {
-            static const int ladder[] = {1, 2, 3, 4, 6, 8};
+            static const int ladder[] = {1, 2, 3, 4, 5, 6, 8};
             for (int c : ladder)
                 if (c >= left) { kk = c; break; }
         }

@@ -43,17 +43,22 @@ void choose_launch(int width, int height, int* kernel, int* k) {
             return;
         }
     }
-    // MI355X sweep (cell-updates/s; tiles 64x64 vs 128x64 at the best k):
+    // MI355X sweep with rain (cell-updates/s; tiles 64x64 vs 128x64 at the best k):
     //   <= 512^2   latency-bound; 64x64 tiles spread the grid over more CUs:
-    //              256^2 2.5e10 (64x64, k = 8) vs 1.6e10 (128x64, k = 8)
-    //   <= 2048^2  2048^2 2.6e11 (64x64, k = 4) vs 2.4e11 (128x64, k = 6)
-    //   larger     4096^2 3.2e11 (128x64, k = 4) vs 3.0e11 (64x64, k = 4)
+    //              256^2 2.4e10 (64x64, k = 8) vs 1.4e10 (128x64, k = 6),
+    //              512^2 8.9e10 (64x64, k = 8) vs 5.5e10 (128x64, k = 6)
+    //   <= 4096^2  1024^2 1.75e11 (128x64, k = 5) vs 1.72e11 (64x64, k = 4),
+    //              2048^2 2.57e11 (128x64, k = 5) vs 2.50e11 (64x64, k = 4),
+    //              4096^2 3.28e11 (128x64, k = 5) vs 3.26e11 (128x64, k = 4)
+    //   larger     16384^2 4.30e11 (128x64, k = 4) vs 4.10e11 (k = 5).  With many
+    //              rounds of work groups per launch the per-launch fill and
+    //              drain matter less than the halo recompute, which grows with k.
     if (cells <= 512 * 512) {
         *kernel = TWS_KERNEL_TILE_64x64;
         *k = 8;
-    } else if (cells <= 2048 * 2048) {
-        *kernel = TWS_KERNEL_TILE_64x64;
-        *k = 4;
+    } else if (cells <= 4096 * 4096) {
+        *kernel = TWS_KERNEL_TILE_128x64;
+        *k = 5;
     } else {
         *kernel = TWS_KERNEL_TILE_128x64;
         *k = 4;
@@ -99,7 +104,7 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
     }
     static const int kernels[] = {TWS_KERNEL_TILE_128x64, TWS_KERNEL_TILE_128x128,
                                   TWS_KERNEL_TILE_256x64, TWS_KERNEL_TILE_64x64};
-    static const int ks[] = {2, 3, 4, 6, 8};
+    static const int ks[] = {2, 3, 4, 5, 6, 8};
     tws_tune_entry top{0, 0, 0.0};
     int m = 0;
     for (int kern : kernels) {

@@ -64,16 +64,36 @@ def small(steps=1000, repeats=5, log=print):
             "volume_m3": vol}
 
 
-def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
+def _draw_sides(rng, n, lo, hi):
+    return np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), n))).astype(int)
+
+
+def batch_sides(n_grids=256, seed=3):
+    """Sides of the batched workload's grids (log-uniform 128..2048)."""
+    return _draw_sides(np.random.default_rng(seed), n_grids, 128, 2048)
+
+
+def batch_grids(n_grids=256, seed=3, only=None):
+    """The batched workload's grids: (side, terrain, rain disc) per grid; sides
+    as batch_sides, terrain seed splitmix(seed * 1000003 + i), rain disc of
+    radius U[8, 32] at a uniform centre, 1e-3 m/step.  `only` picks grid
+    indices (the terrain of the others is not generated)."""
     rng = np.random.default_rng(seed)
-    sides = np.round(np.exp(rng.uniform(math.log(128), math.log(2048), n_grids))).astype(int)
-    p = tws.params_physical(**PARAMS)
+    sides = _draw_sides(rng, n_grids, 128, 2048)
     grids = []
     for i, s in enumerate(sides):
-        b = tws.generate_fbm_terrain(int(s), int(s), seed=_splitmix(seed * 1000003 + i))
         r = float(rng.uniform(8, 32))
         rain = (float(rng.uniform(0, s)), float(rng.uniform(0, s)), 1e-3, r * r)
+        if only is not None and i not in only:
+            continue
+        b = tws.generate_fbm_terrain(int(s), int(s), seed=_splitmix(seed * 1000003 + i))
         grids.append((int(s), b, rain))
+    return grids
+
+
+def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
+    p = tws.params_physical(**PARAMS)
+    grids = batch_grids(n_grids, seed)
     cells = int(sum(s * s for s, _, _ in grids))
     log(f"batched: {n_grids} grids, {cells / 1e6:.1f} M cells; autotuning")
     best, trials = tws.autotune([(s, s) for s, _, _ in grids], steps=50)
@@ -135,12 +155,12 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
 
 
 # ---------------------------------------------------------------------------
-class _Requests:
+class Requests:
     """The served request stream: terrain windows cut from one fBm field."""
 
     def __init__(self, n, seed, rate):
         rng = np.random.default_rng(seed)
-        self.sides = np.round(np.exp(rng.uniform(math.log(256), math.log(4096), n))).astype(int)
+        self.sides = _draw_sides(rng, n, 256, 4096)
         self.steps = rng.integers(100, 5001, n)
         self.kind = rng.integers(0, 2, n)  # 0 dam-break, 1 rain
         self.arrival = np.cumsum(rng.exponential(1.0 / rate, n))
@@ -176,10 +196,10 @@ def _pcts(lat):
 
 def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log=print):
     p = tws.params_physical(**PARAMS)
-    probe = _Requests(n_requests, seed, 1.0)
+    probe = Requests(n_requests, seed, 1.0)
     mean_work = probe.work() / n_requests
     rate = load * capacity_cups / mean_work
-    req = _Requests(n_requests, seed, rate)
+    req = Requests(n_requests, seed, rate)
     log(f"served: {n_requests} requests, mean work {mean_work:.3g} cell-updates, "
         f"rate {rate:.2f}/s ({load:.0%} of {capacity_cups:.3g} cell-updates/s)")
     states = [req.state(i) for i in range(n_requests)]

@@ -143,3 +143,37 @@ def test_nan_in_one_grid_is_isolated(gpu):
         assert not batch.check_finite(ids[1])
         b, d, f, p, rain = grids[0]
         assert np.array_equal(batch.download(ids[0])["depth"], alone(gpu, b, d, f, p, rain, 5)["depth"])
+
+
+def test_workload_grids_vs_restatements(gpu, oracle):
+    """Grids of the batched workload (its 4 smallest sides, the median and the
+    largest of 256), 100 steps of rain in one batch: every grid bit-identical
+    to the FP32 restatement; FP32 vs FP64 error reported per grid."""
+    from terrainwatersim_amd.workloads import PARAMS, batch_grids, batch_sides
+    order = np.argsort(batch_sides(), kind="stable")
+    pick = [int(i) for i in list(order[:4]) + [order[128], order[-1]]]
+    grids = batch_grids(only=set(pick))
+    p = gpu.params_physical(**PARAMS)
+    pool = sum(gpu.grid_footprint(s, s) for s, _, _ in grids)
+    oracle.set_threads(16)
+    with gpu.Batch(len(grids), pool) as batch:
+        ids = []
+        for s, b, rain in grids:
+            gid = batch.add(s, s, p)
+            batch.upload(gid, b, np.zeros((s, s), F32))
+            batch.set_rain(gid, *rain)
+            ids.append(gid)
+        batch.step(100)
+        outs = [batch.download(g, flux=True) for g in ids]
+    F, K, Cc = p.flow_friction, p.water_acceleration, p.cell_area_inv_timescaled
+    for (s, b, rain), out in zip(grids, outs):
+        d0 = np.zeros((s, s), F32)
+        d32, f32, _ = oracle.step(b, d0, None, F, K, Cc, n=100, rain=rain)
+        assert np.array_equal(out["depth"], d32) and np.array_equal(out["flux"], f32), s
+        d64, _, _ = oracle.step(b, d0, None, F, K, Cc, n=100, rain=rain, dtype=np.float64)
+        m = float(d64.max())
+        err = np.abs(out["depth"].astype(np.float64) - d64)
+        print(f"batched grid {s}^2, rain R^2={rain[3]:.0f}, 100 steps: FP32 vs FP64 max abs "
+              f"{err.max():.3e} m, max rel (floor 1e-3*max) "
+              f"{(err / np.maximum(d64, 1e-3 * m)).max():.3e}, max depth {m:.3g} m")
+        assert err.max() < 1e-4 * m

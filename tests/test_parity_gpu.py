@@ -55,8 +55,8 @@ def test_lane_shift_layout(gpu):
     assert out[64:].tolist() == list(range(2, 65)) + [0]
 
 
-KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 8), (4, 1), (4, 2), (4, 4), (4, 8),
-           (5, 1), (5, 4), (5, 6), (5, 8)]
+KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 5), (3, 8), (4, 1), (4, 2), (4, 4),
+           (4, 5), (4, 8), (5, 1), (5, 4), (5, 5), (5, 6), (5, 8)]
 
 
 @pytest.mark.parametrize("kernel,k", KERNELS)
@@ -162,6 +162,44 @@ def test_full_size_4096_properties(gpu, oracle):
         assert s.check_finite()
     assert (dd >= 0).all()
     assert abs(v1 - v0) / v0 < 1e-6, (v0, v1)
+
+
+def fp64_report(label, got, r64):
+    """Prints and returns (max abs, max rel with a floor of 1e-3 * max depth)
+    of FP32 depth `got` against the FP64 restatement `r64`."""
+    m = float(r64.max())
+    err = np.abs(got.astype(np.float64) - r64)
+    rel = float((err / np.maximum(np.abs(r64), 1e-3 * m)).max())
+    print(f"{label}: FP32 vs FP64 max abs {err.max():.3e} m, max rel (floor 1e-3*max) "
+          f"{rel:.3e}, max depth {m:.3g} m")
+    return float(err.max()), rel, m
+
+
+def test_full_size_4096_window_vs_restatements(gpu, oracle):
+    """The 4096^2 sheet workload with the default launch: the centre 1024^2
+    window after 256 steps is bit-identical to the FP32 restatement run on the
+    2052^2 sub-domain that holds the window's dependency cone (2 cells per
+    step); FP32 vs FP64 error reported on the same window."""
+    N, window, t = 4096, 1024, 256
+    b = gpu.generate_fbm_terrain(N, N, seed=2)
+    d = np.full((N, N), 0.5, F32)
+    p = gpu.params_physical(**CFG_PARAMS)
+    w0 = (N - window) // 2
+    with gpu.Simulation(N, N, p) as s:
+        s.upload(b, d)
+        s.step(t)
+        win = s.download(flux=False)["depth"][w0:w0 + window, w0:w0 + window].copy()
+    pad = 2 * t + 2
+    s0, s1 = w0 - pad, w0 + window + pad
+    bs = np.ascontiguousarray(b[s0:s1, s0:s1])
+    ds = np.ascontiguousarray(d[s0:s1, s0:s1])
+    oracle.set_threads(16)
+    d32, _, _ = oracle.step(bs, ds, None, *p3(p), n=t)
+    assert np.array_equal(win, d32[pad:pad + window, pad:pad + window])
+    d64, _, _ = oracle.step(bs, ds, None, *p3(p), n=t, dtype=np.float64)
+    err, _, m = fp64_report(f"4096^2 sheet, centre {window}^2 after {t} steps", win,
+                            d64[pad:pad + window, pad:pad + window])
+    assert err < 1e-4 * m
 
 
 def test_split_run_equals_continuous(gpu):

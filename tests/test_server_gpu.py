@@ -126,3 +126,38 @@ def test_release_forgets_finished_requests_only(gpu):
         assert srv.poll(t_long).state == gpu.REQ_CANCELLED
         srv.release(t_long)
         assert srv.stats()["completed"] == 1 and srv.stats()["cancelled"] == 1
+
+
+def test_workload_requests_vs_restatements(gpu, oracle):
+    """Requests of the served workload (the smallest dam-break and the smallest
+    rain solve of its 200), through the server: bit-identical to the FP32
+    restatement over their full step counts; FP32 vs FP64 error reported."""
+    from terrainwatersim_amd.workloads import PARAMS, Requests
+    req = Requests(200, 4, 1.0)
+    pick = []
+    for kind in (0, 1):
+        idx = [i for i in range(200) if req.kind[i] == kind]
+        pick.append(min(idx, key=lambda i: (req.sides[i], i)))
+    p = gpu.params_physical(**PARAMS)
+    F, K, Cc = p.flow_friction, p.water_acceleration, p.cell_area_inv_timescaled
+    oracle.set_threads(16)
+    with gpu.Server(2 * GB, max_resident=8, chunk_steps=100) as srv:
+        sub = []
+        for i in pick:
+            b, d, rain = req.state(i)
+            t, out = srv.submit(b, int(req.steps[i]), p, depth=d, rain=rain)
+            sub.append((i, t, out, b, d, rain))
+        srv.drain()
+        for i, t, out, b, d, rain in sub:
+            n = int(req.steps[i])
+            assert srv.poll(t).state == gpu.REQ_DONE
+            d32, f32, _ = oracle.step(b, d, None, F, K, Cc, n=n, rain=rain)
+            assert np.array_equal(out["depth"], d32) and np.array_equal(out["flux"], f32), i
+            d64, _, _ = oracle.step(b, d, None, F, K, Cc, n=n, rain=rain, dtype=np.float64)
+            m = float(d64.max())
+            err = np.abs(out["depth"].astype(np.float64) - d64)
+            kind = "dam-break" if rain is None else "rain"
+            print(f"served request {i}: {kind} {b.shape[0]}^2, {n} steps: FP32 vs FP64 max abs "
+                  f"{err.max():.3e} m, max rel (floor 1e-3*max) "
+                  f"{(err / np.maximum(d64, 1e-3 * m)).max():.3e}, max depth {m:.3g} m")
+            assert err.max() < 1e-2 * m

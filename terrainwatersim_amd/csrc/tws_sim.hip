@@ -212,14 +212,18 @@ int tws_step_profiled(tws_sim* s, int32_t n, float* launch_ms, int32_t capacity,
 int tws_perform_simulation_step(tws_sim* s, double frame_seconds, double step_seconds,
                                 int32_t* steps_done) {
     if (!s) return fail(TWS_ERR_INVALID, "tws_perform_simulation_step: NULL handle");
-    if (!(step_seconds > 0.0) || !(frame_seconds >= 0.0))
+    if (!(step_seconds > 0.0) || !(frame_seconds >= 0.0) || !std::isfinite(step_seconds) ||
+        !std::isfinite(frame_seconds))
         return fail(TWS_ERR_INVALID, "tws_perform_simulation_step: invalid times");
     // Terrain.cpp:242-247: accumulate, take whole steps, then clamp to 10
-    // (time beyond the clamp is dropped, as in the reference).
+    // (time beyond the clamp is dropped, as in the reference).  The whole
+    // step count is taken in double, so a long frame cannot overflow the
+    // integer conversion; rounding can leave the accumulator an ulp below
+    // zero, which counts as no step.
     s->time_acc += frame_seconds;
-    uint32_t n = (uint32_t)(s->time_acc / step_seconds);
-    s->time_acc -= step_seconds * n;
-    if (n > 10) n = 10;
+    const double whole = std::fmax(0.0, std::floor(s->time_acc / step_seconds));
+    s->time_acc -= step_seconds * whole;
+    const uint32_t n = whole > 10.0 ? 10u : (uint32_t)whole;
     if (steps_done) *steps_done = (int32_t)n;
     const int rc = tws_step(s, (int32_t)n);
     if (rc != TWS_OK || n == 0 || !s->frame_mips) return rc;

@@ -264,3 +264,8 @@ def test_perform_simulation_step_clamps_like_reference(gpu):
         assert s.perform_simulation_step(0.05, 1.0 / 60.0) == 3
         assert s.perform_simulation_step(1.0, 1.0 / 60.0) == 10  # clamp; time dropped
         assert s.perform_simulation_step(0.0, 1.0 / 60.0) == 0
+        assert s.perform_simulation_step(1e300, 1.0 / 60.0) == 10  # no integer overflow
+        assert s.perform_simulation_step(0.0, 1.0 / 60.0) == 0
+        for bad in (float("inf"), float("nan"), -1.0):
+            with pytest.raises(gpu.TwsError):
+                s.perform_simulation_step(bad, 1.0 / 60.0)

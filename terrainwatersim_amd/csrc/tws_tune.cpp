@@ -45,11 +45,11 @@ void choose_launch(int width, int height, int* kernel, int* k) {
     }
     // MI355X sweep with rain (cell-updates/s; tiles 64x64 vs 128x64 at the best k):
     //   <= 512^2   latency-bound; 64x64 tiles spread the grid over more CUs:
-    //              256^2 2.4e10 (64x64, k = 8) vs 1.4e10 (128x64, k = 6),
-    //              512^2 8.9e10 (64x64, k = 8) vs 5.5e10 (128x64, k = 6)
-    //   <= 4096^2  1024^2 1.75e11 (128x64, k = 5) vs 1.72e11 (64x64, k = 4),
-    //              2048^2 2.57e11 (128x64, k = 5) vs 2.50e11 (64x64, k = 4),
-    //              4096^2 3.28e11 (128x64, k = 5) vs 3.26e11 (128x64, k = 4)
+    //              256^2 2.5e10 (64x64, k = 8) vs 1.5e10 (128x64, k = 6),
+    //              512^2 9.2e10 (64x64, k = 8) vs 5.6e10 (128x64, k = 6)
+    //   <= 4096^2  1024^2 1.77e11 (128x64, k = 5) vs 1.74e11 (64x64, k = 4),
+    //              2048^2 2.58e11 (128x64, k = 5) vs 2.54e11 (64x64, k = 4),
+    //              4096^2 3.32e11 (128x64, k = 5) vs 3.30e11 (128x64, k = 4)
     //   larger     16384^2 4.30e11 (128x64, k = 4) vs 4.10e11 (k = 5).  With many
     //              rounds of work groups per launch the per-launch fill and
     //              drain matter less than the halo recompute, which grows with k.

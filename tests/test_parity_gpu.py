@@ -70,6 +70,30 @@ def test_tile_kernel_bitwise(gpu, oracle, kernel, k):
     assert_same(out, de, fe, ve)
 
 
+RAIN_DISCS = [(100.0, 60.0, 1e-3, 256.0),     # inside
+              (0.5, 199.5, 2e-3, 900.0),       # centre on a corner, most of the disc outside
+              (111.9, 47.9, 1e-2, 0.5),        # under one cell, next to tile seams
+              (-30.0, 80.0, 1e-3, 1024.0),     # centre outside, 2 columns inside
+              (150.0, 100.0, 1e-4, 1e30),      # covers the whole grid
+              (5000.0, -700.0, 1e-3, 400.0),   # far outside: no cell
+              (64.25, 56.75, 5e-3, 3.0)]       # a few cells at a wave's first rows
+
+
+@pytest.mark.parametrize("kernel,k", [(3, 4), (3, 5), (5, 8), (2, 3), (4, 6)])
+def test_rain_disc_positions(gpu, oracle, kernel, k):
+    """The step kernels skip the brush for waves that the disc cannot reach; a
+    disc at every kind of position (under a single cell, across tile seams,
+    over a corner, partly or wholly outside, covering everything) gives the
+    restatement's depth and flux bit for bit."""
+    b, d, f = random_state(77 + kernel * 10 + k, 200, 300)
+    p = gpu.params_physical(**CFG_PARAMS)
+    for rain in RAIN_DISCS:
+        out = run_gpu(gpu, b, d, f, 9, kernel, k, p, rain=rain, velocity=False)
+        de, fe, _ = oracle.step(b, d, f, *p3(p), n=9, rain=rain)
+        assert np.array_equal(out["depth"], de), rain
+        assert np.array_equal(out["flux"], fe), rain
+
+
 def test_twopass_bitwise(gpu, oracle):
     b, d, f = random_state(3, 90, 70)
     p = gpu.params_physical(**CFG_PARAMS)

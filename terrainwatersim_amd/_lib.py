@@ -30,7 +30,9 @@ KERNEL_TILE_128x128 = 2
 KERNEL_TILE_128x64 = 3
 KERNEL_TILE_256x64 = 4
 KERNEL_TILE_64x64 = 5
-TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64, KERNEL_TILE_64x64)
+KERNEL_TILE_64x64_16W = 6
+TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64, KERNEL_TILE_64x64,
+                KERNEL_TILE_64x64_16W)
 
 
 class TwsError(RuntimeError):

@@ -44,9 +44,12 @@ void choose_launch(int width, int height, int* kernel, int* k) {
         }
     }
     // MI355X sweep with rain (cell-updates/s; tiles 64x64 vs 128x64 at the best k):
-    //   <= 512^2   latency-bound; 64x64 tiles spread the grid over more CUs:
-    //              256^2 2.5e10 (64x64, k = 8) vs 1.5e10 (128x64, k = 6),
-    //              512^2 9.2e10 (64x64, k = 8) vs 5.6e10 (128x64, k = 6)
+    //   <= 512^2   latency-bound: 64x64 tiles spread the grid over more CUs, and
+    //              16 waves of 4 rows (instead of 8 of 8) halve each wave's
+    //              serial work per level:
+    //              256^2 2.72e10 (64x64 16 waves, k = 8) vs 2.47e10 (8 waves)
+    //                    vs 1.5e10 (128x64, k = 6),
+    //              512^2 1.00e11 (64x64 16 waves, k = 8) vs 9.3e10 (8 waves)
     //   <= 4096^2  1024^2 1.77e11 (128x64, k = 5) vs 1.74e11 (64x64, k = 4),
     //              2048^2 2.58e11 (128x64, k = 5) vs 2.54e11 (64x64, k = 4),
     //              4096^2 3.32e11 (128x64, k = 5) vs 3.30e11 (128x64, k = 4)
@@ -54,7 +57,7 @@ void choose_launch(int width, int height, int* kernel, int* k) {
     //              rounds of work groups per launch the per-launch fill and
     //              drain matter less than the halo recompute, which grows with k.
     if (cells <= 512 * 512) {
-        *kernel = TWS_KERNEL_TILE_64x64;
+        *kernel = TWS_KERNEL_TILE_64x64_16W;
         *k = 8;
     } else if (cells <= 4096 * 4096) {
         *kernel = TWS_KERNEL_TILE_128x64;
@@ -103,7 +106,8 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
         if (rc == TWS_OK) rc = tws_batch_upload(b, id, terrain.data(), depth.data(), nullptr);
     }
     static const int kernels[] = {TWS_KERNEL_TILE_128x64, TWS_KERNEL_TILE_128x128,
-                                  TWS_KERNEL_TILE_256x64, TWS_KERNEL_TILE_64x64};
+                                  TWS_KERNEL_TILE_256x64, TWS_KERNEL_TILE_64x64,
+                                  TWS_KERNEL_TILE_64x64_16W};
     static const int ks[] = {2, 3, 4, 5, 6, 8};
     tws_tune_entry top{0, 0, 0.0};
     int m = 0;

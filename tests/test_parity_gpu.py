@@ -56,7 +56,7 @@ def test_lane_shift_layout(gpu):
 
 
 KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 5), (3, 8), (4, 1), (4, 2), (4, 4),
-           (4, 5), (4, 8), (5, 1), (5, 4), (5, 5), (5, 6), (5, 8)]
+           (4, 5), (4, 8), (5, 1), (5, 4), (5, 5), (5, 6), (5, 8), (6, 1), (6, 3), (6, 5), (6, 8)]
 
 
 @pytest.mark.parametrize("kernel,k", KERNELS)

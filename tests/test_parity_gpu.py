@@ -102,7 +102,7 @@ def test_twopass_bitwise(gpu, oracle):
     assert_same(out, de, fe, ve)
 
 
-@pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 2), (4, 8)])
+@pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 2), (4, 8), (3, 5), (5, 4), (6, 8)])
 def test_zero_boundary_bitwise(gpu, oracle, kernel, k):
     b, d, f = random_state(21, 130, 259)
     p = gpu.params_physical(**CFG_PARAMS, boundary=gpu.BOUNDARY_ZERO)
@@ -111,12 +111,13 @@ def test_zero_boundary_bitwise(gpu, oracle, kernel, k):
     assert_same(out, de, fe, ve)
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (5, 1), (2, 3), (64, 64), (129, 257)])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (5, 1), (2, 3), (64, 64), (129, 257),
+                                   (1, 5000), (3000, 2), (45, 109)])
 def test_ragged_and_tiny_grids(gpu, oracle, shape):
     H, W = shape
     b, d, f = random_state(H * 1000 + W, H, W)
     p = gpu.params_physical(**CFG_PARAMS)
-    for kernel, k in ((2, 4), (4, 8), (1, 1)):
+    for kernel, k in ((2, 4), (4, 8), (1, 1), (3, 5), (5, 6), (6, 8)):
         out = run_gpu(gpu, b, d, f, 6, kernel, k, p)
         de, fe, ve = oracle.step(b, d, f, *p3(p), n=6, velocity=True)
         assert_same(out, de, fe, ve)

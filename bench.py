@@ -125,6 +125,15 @@ def main():
     bytes_per_launch = BYTES_PER_CELL_UPDATE * cells * k
     achieved = bytes_per_launch / (mean_ms * 1e-3)
     finite = sim.check_finite()
+    # The same steps with the boundary's host transfers inside the clock:
+    # upload b and d from (pageable) host arrays, step, download d and 4 f.
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.upload(b, d)
+    sim.step(args.steps)
+    host_out = sim.download(flux=True)
+    el_io = time.perf_counter() - t0
+    del host_out
     sim.close()
     copy_gbps = tws.stream_copy_gbps(1 << 30, 20)
 
@@ -169,6 +178,13 @@ def main():
             "stream_copy_GBps_measured": copy_gbps,
         },
         "finite": finite,
+        "pcie_inclusive": {
+            "cell_updates_per_s": cells * args.steps / el_io,
+            "seconds": el_io,
+            "compute_only_seconds": el,
+            "transfers": "upload b, d from host arrays (flux zeroed on the device), "
+                         f"{args.steps} steps, download d and the 4 flux planes",
+        },
     }
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (FP32 restatement on host cores)")

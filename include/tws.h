@@ -310,7 +310,8 @@ typedef struct tws_tune_entry {
 
 /* Times every (tile variant, steps per launch) candidate on scratch grids of
  * the n given shapes advanced together as one batch (synthetic state: seeded
- * fBm terrain under a 0.5 m water sheet), `steps` steps per trial, and
+ * fBm terrain under a 0.5 m water sheet, a rain disc of radius 16 at each
+ * grid's centre), `steps` steps per trial after an untimed run of as many, and
  * returns the fastest in *best.  results[0..capacity) (may be NULL) receives
  * every trial; *count (may be NULL) their number.  The winner is recorded for
  * the size class (log2 of cells) of the average grid, and TWS_KERNEL_AUTO

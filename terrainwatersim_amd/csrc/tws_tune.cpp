@@ -2,13 +2,14 @@
 // table and the measuring autotuner.
 //
 // choose_launch() maps a grid size to (kernel variant, steps per launch).
-// The built-in table comes from an MI355X sweep over sizes 128^2..4096^2:
-// 64 x 64 tiles up to 2048^2 (small grids are bound by the latency of one
-// tile's levels, so more and smaller tiles finish sooner), 128 x 64 tiles
-// beyond (two work groups per CU so one loads while the other computes,
-// 4 steps per launch: 3.3e11 cell-updates/s at 4096^2).  tws_autotune() times every candidate on
-// scratch grids of the requested shapes and records the winner for that
-// size class (log2 of the cell count), which choose_launch() then prefers.
+// The built-in table comes from an MI355X sweep over sizes 128^2..16384^2:
+// 64 x 64 tiles over 16 waves up to 512^2 (small grids are bound by the
+// latency of one tile's levels, so more and smaller tiles finish sooner),
+// 128 x 64 tiles beyond (two work groups per CU so one loads while the other
+// computes), 5 steps per launch up to 4096^2 and 4 above.  tws_autotune()
+// times every candidate on scratch grids of the requested shapes and records
+// the winner for that size class (log2 of the cell count), which
+// choose_launch() then prefers.
 
 #include <hip/hip_runtime.h>
 
@@ -92,7 +93,9 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
     tws_batch* b = nullptr;
     rc = tws_batch_create(n, bytes, nullptr, &b);
     if (rc != TWS_OK) return rc;
-    // synthetic wet state: fBm terrain 0..100 m under a 0.5 m sheet
+    // synthetic wet state: fBm terrain 0..100 m under a 0.5 m sheet, and a
+    // rain disc (r = 16) at the centre as in the batched and served
+    // workloads, so the kernel timed is the one with the rain source
     std::vector<float> terrain, depth;
     for (int i = 0; i < n && rc == TWS_OK; ++i) {
         int32_t id = -1;
@@ -104,6 +107,8 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
         rc = tws_generate_fbm_terrain(widths[i], heights[i], 1000u + (uint64_t)i, 6, 4, 0.43f,
                                       100.0f, terrain.data());
         if (rc == TWS_OK) rc = tws_batch_upload(b, id, terrain.data(), depth.data(), nullptr);
+        if (rc == TWS_OK)
+            rc = tws_batch_set_rain(b, id, 0.5f * widths[i], 0.5f * heights[i], 1e-3f, 256.0f);
     }
     static const int kernels[] = {TWS_KERNEL_TILE_128x64, TWS_KERNEL_TILE_128x128,
                                   TWS_KERNEL_TILE_256x64, TWS_KERNEL_TILE_64x64,
@@ -115,7 +120,9 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
         for (int kk : ks) {
             if (rc != TWS_OK) break;
             rc = tws_batch_set_launch(b, kern, kk);
-            if (rc == TWS_OK) rc = tws_batch_step(b, kk);  // warm: capture + first replay
+            // warm with the same step count: the timed run then replays the
+            // captured launch sequence instead of capturing one
+            if (rc == TWS_OK) rc = tws_batch_step(b, steps);
             if (rc == TWS_OK) rc = tws_batch_sync(b);
             const auto t0 = std::chrono::steady_clock::now();
             if (rc == TWS_OK) rc = tws_batch_step(b, steps);

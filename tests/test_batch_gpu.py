@@ -177,3 +177,27 @@ def test_workload_grids_vs_restatements(gpu, oracle):
               f"{err.max():.3e} m, max rel (floor 1e-3*max) "
               f"{(err / np.maximum(d64, 1e-3 * m)).max():.3e}, max depth {m:.3g} m")
         assert err.max() < 1e-4 * m
+
+
+def test_autotune_records_the_fastest_candidate(gpu):
+    """tws_autotune times every (variant, steps per launch) candidate, returns
+    the fastest, and TWS_KERNEL_AUTO picks it for grids of that size class
+    until tws_tune_forget restores the built-in table."""
+    shapes = [(300, 280), (260, 310), (290, 290)]
+    p = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995)
+    with gpu.Simulation(290, 290, p) as s:
+        builtin = s.get_launch()
+    try:
+        best, trials = gpu.autotune(shapes, steps=16)
+        assert len(trials) == len(gpu.TILE_KERNELS) * 6
+        assert {(t["kernel"], t["k"]) for t in trials} == {
+            (kern, k) for kern in gpu.TILE_KERNELS for k in (2, 3, 4, 5, 6, 8)}
+        assert all(t["cell_updates_per_s"] > 0 for t in trials)
+        top = max(trials, key=lambda t: t["cell_updates_per_s"])
+        assert (best["kernel"], best["k"]) == (top["kernel"], top["k"])
+        with gpu.Simulation(290, 290, p) as s:
+            assert s.get_launch() == (best["kernel"], best["k"])
+    finally:
+        gpu.tune_forget()
+    with gpu.Simulation(290, 290, p) as s:
+        assert s.get_launch() == builtin

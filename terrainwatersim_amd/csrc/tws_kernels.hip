@@ -269,8 +269,18 @@ __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int
             h_up[c] = wave > 0 ? s_xh[1][wave - 1][c][lane] : h[0][c];
             h_dn[c] = wave < NW - 1 ? s_xh[0][wave + 1][c][lane] : h[RPW - 1][c];
         }
+        // Rows that can still reach the output: the flux of level lev is
+        // needed on rows [2 lev + 1, EXH - 2 lev - 1), the depth on
+        // [2 lev + 2, EXH - 2 lev - 2).  Rows outside only feed rows outside,
+        // so a wave may skip them (wave-uniform branches).  Measured: +12-14%
+        // for 64x64 tiles at 256^2-512^2, +2-5% for the general kernel at
+        // 2048^2-4096^2, but -2% for the plain kernels whose lanes hold 16
+        // cells (the branches cost more than the rows they save there).
+        constexpr bool kSkipRows = GEN || CPL * RPW < 16;
+        const int flo = 2 * lev + 1 - ly, fhi = EXH - 2 * lev - 1 - ly;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
+            if (kSkipRows && (r < flo || r >= fhi)) continue;
             const float h_left = from_left_lane_or(h[r][CPL - 1], h[r][0]);
             const float h_right = from_right_lane_or(h[r][0], h[r][CPL - 1]);
 #pragma unroll
@@ -310,6 +320,7 @@ __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int
         }
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
+            if (kSkipRows && (r < flo + 1 || r >= fhi - 1)) continue;
             const float in_left = from_left_lane(f0[r][CPL - 1]);   // X0.x
             const float in_right = from_right_lane(f1[r][0]);      // X1.y
 #pragma unroll

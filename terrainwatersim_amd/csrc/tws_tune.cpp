@@ -48,9 +48,9 @@ void choose_launch(int width, int height, int* kernel, int* k) {
     //   <= 512^2   latency-bound: 64x64 tiles spread the grid over more CUs, and
     //              16 waves of 4 rows (instead of 8 of 8) halve each wave's
     //              serial work per level:
-    //              256^2 2.72e10 (64x64 16 waves, k = 8) vs 2.47e10 (8 waves)
+    //              256^2 3.07e10 (64x64 16 waves, k = 8) vs 2.54e10 (8 waves)
     //                    vs 1.5e10 (128x64, k = 6),
-    //              512^2 1.00e11 (64x64 16 waves, k = 8) vs 9.3e10 (8 waves)
+    //              512^2 1.14e11 (64x64 16 waves, k = 8) vs 9.9e10 (8 waves)
     //   <= 4096^2  1024^2 1.77e11 (128x64, k = 5) vs 1.74e11 (64x64, k = 4),
     //              2048^2 2.58e11 (128x64, k = 5) vs 2.54e11 (64x64, k = 4),
     //              4096^2 3.32e11 (128x64, k = 5) vs 3.30e11 (128x64, k = 4)

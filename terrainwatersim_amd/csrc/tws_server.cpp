@@ -363,7 +363,8 @@ int tws_server_submit(tws_server* s, const tws_request* r, uint64_t* ticket) {
     if (!s || !r || !ticket) return fail(TWS_ERR_INVALID, "tws_server_submit: NULL argument");
     if (r->width <= 0 || r->height <= 0 || r->steps < 0 || !r->terrain || !valid_params(&r->params))
         return fail(TWS_ERR_INVALID, "tws_server_submit: invalid request");
-    if (r->rain_on && (!(r->rain[3] > 0.0f) || !std::isfinite(r->rain[2])))
+    if (r->rain_on && (!(r->rain[3] > 0.0f) || !std::isfinite(r->rain[0]) ||
+                       !std::isfinite(r->rain[1]) || !std::isfinite(r->rain[2])))
         return fail(TWS_ERR_INVALID, "tws_server_submit: invalid rain");
     const int64_t fp = tws_grid_footprint(r->width, r->height);
     if (fp > s->stats.pool_bytes)

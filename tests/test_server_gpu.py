@@ -100,6 +100,10 @@ def test_cancel_and_nonfinite_are_isolated(gpu):
         t_bad, _ = srv.submit(b, 60, p, depth=bad)
         t_cancel, _ = srv.submit(b, 100000, p, depth=d)
         srv.cancel(t_cancel)
+        for rain in ((np.nan, 5.0, 1e-3, 16.0), (5.0, np.inf, 1e-3, 16.0), (5.0, 5.0, np.nan, 16.0),
+                     (5.0, 5.0, 1e-3, 0.0)):
+            with pytest.raises(gpu.TwsError):   # refused at submit
+                srv.submit(b, 10, p, depth=d, rain=rain)
         srv.drain()
         assert srv.poll(t_ok).state == gpu.REQ_DONE
         st_bad = srv.poll(t_bad)

@@ -53,7 +53,10 @@ extern "C" {
                                    (flowUpdate.comp:14-18, flowApply.comp:16-20) */
 
 /* Per-step constants of the SimulationParameters block
- * (simulationCommon.glsl:1-13) plus the boundary mode. */
+ * (simulationCommon.glsl:1-13) plus the boundary mode.  F, K and C must be
+ * finite and >= 0, as the reference derives them (Terrain.cpp:175-198):
+ * closed walls are exact only while a head difference toward a wall cannot
+ * push water into it (K >= 0) for terrain and depths below ~1e28. */
 typedef struct tws_params {
     float flow_friction;            /* F = FlowFriction_perStep             */
     float water_acceleration;       /* K = WaterAcceleration_perStep        */

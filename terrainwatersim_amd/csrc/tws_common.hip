@@ -36,6 +36,11 @@ bool valid_params(const tws_params* p) {
     if (!std::isfinite(p->flow_friction) || !std::isfinite(p->water_acceleration) ||
         !std::isfinite(p->cell_area_inv_timescaled) || !std::isfinite(p->ghost_alpha))
         return false;
+    // the reference derives all three from non-negative quantities; the
+    // closed-wall ghost (height 1e30) relies on K >= 0
+    if (p->flow_friction < 0.0f || p->water_acceleration < 0.0f ||
+        p->cell_area_inv_timescaled < 0.0f)
+        return false;
     return p->boundary == TWS_BOUNDARY_CLOSED || p->boundary == TWS_BOUNDARY_ZERO;
 }
 

@@ -63,6 +63,14 @@ def test_invalid_arguments_are_rejected(tws):
     assert tws.lib().tws_step(None, 1) == -1
     assert tws.lib().tws_destroy(None) == 0
     del p
+    # F, K, C must be finite and >= 0 (checked before any device call)
+    import ctypes as C
+    h = C.c_void_p()
+    for F, K, Cc in ((0.99, -0.1, 0.02), (-0.5, 0.2, 0.02), (0.99, 0.2, -0.02),
+                     (float("nan"), 0.2, 0.02), (0.99, float("inf"), 0.02)):
+        bad = tws.params_raw(F, K, Cc)
+        assert tws.lib().tws_create(64, 64, C.byref(bad), None, C.byref(h)) == -1
+        assert not h.value
 
 
 def test_fbm_terrain_is_deterministic(tws):

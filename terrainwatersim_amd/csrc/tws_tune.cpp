@@ -3,10 +3,10 @@
 //
 // choose_launch() maps a grid size to (kernel variant, steps per launch).
 // The built-in table comes from an MI355X sweep over sizes 128^2..16384^2:
-// 64 x 64 tiles over 16 waves up to 512^2 (small grids are bound by the
-// latency of one tile's levels, so more and smaller tiles finish sooner),
-// 128 x 64 tiles beyond (two work groups per CU so one loads while the other
-// computes), 5 steps per launch up to 4096^2 and 4 above.  tws_autotune()
+// 64 x 64 tiles (16 waves up to 512^2, 8 waves up to 2048^2: small grids
+// are bound by the latency of one tile's levels, so more and smaller tiles
+// finish sooner), 128 x 64 tiles beyond (two work groups per CU so one loads
+// while the other computes), 5 steps per launch up to 4096^2 and 4 above.  tws_autotune()
 // times every candidate on scratch grids of the requested shapes and records
 // the winner for that size class (log2 of the cell count), which
 // choose_launch() then prefers.
@@ -44,22 +44,25 @@ void choose_launch(int width, int height, int* kernel, int* k) {
             return;
         }
     }
-    // MI355X sweep with rain (cell-updates/s; tiles 64x64 vs 128x64 at the best k):
+    // MI355X sweep (cell-updates/s; 0.5 m sheet on fBm terrain, with and without
+    // a rain disc; stale halo rows skipped in the 64-wide tiles):
     //   <= 512^2   latency-bound: 64x64 tiles spread the grid over more CUs, and
     //              16 waves of 4 rows (instead of 8 of 8) halve each wave's
-    //              serial work per level:
-    //              256^2 3.07e10 (64x64 16 waves, k = 8) vs 2.54e10 (8 waves)
-    //                    vs 1.5e10 (128x64, k = 6),
-    //              512^2 1.14e11 (64x64 16 waves, k = 8) vs 9.9e10 (8 waves)
-    //   <= 4096^2  1024^2 1.77e11 (128x64, k = 5) vs 1.74e11 (64x64, k = 4),
-    //              2048^2 2.58e11 (128x64, k = 5) vs 2.54e11 (64x64, k = 4),
-    //              4096^2 3.32e11 (128x64, k = 5) vs 3.30e11 (128x64, k = 4)
-    //   larger     16384^2 4.30e11 (128x64, k = 4) vs 4.10e11 (k = 5).  With many
-    //              rounds of work groups per launch the per-launch fill and
-    //              drain matter less than the halo recompute, which grows with k.
+    //              serial work per level: 512^2 1.16e11 (64x64 16 waves, k = 8)
+    //              vs 8.7e10 (8 waves, k = 6) vs 5.1e10 (128x64, k = 5)
+    //   <= 2048^2  64x64 tiles, 8 waves, k = 4: 2048^2 2.98e11 with rain
+    //              (3.31e11 without) vs 2.67e11 (2.91e11) for 128x64, k = 5;
+    //              1024^2 1.88e11 with rain vs 1.72e11
+    //   <= 4096^2  128x64, k = 5: 3.36e11 with rain (3.59e11 without) vs
+    //              3.08e11 (3.42e11) for 64x64, k = 4
+    //   larger     128x64, k = 4: the 16384^2 dam-break 4.30e11 vs 4.10e11 at
+    //              k = 5 (on an all-wet 16384^2 sheet k = 5 leads by 4%).
     if (cells <= 512 * 512) {
         *kernel = TWS_KERNEL_TILE_64x64_16W;
         *k = 8;
+    } else if (cells <= 2048 * 2048) {
+        *kernel = TWS_KERNEL_TILE_64x64;
+        *k = 4;
     } else if (cells <= 4096 * 4096) {
         *kernel = TWS_KERNEL_TILE_128x64;
         *k = 5;

@@ -318,8 +318,10 @@ typedef struct tws_tune_entry {
  * returns the fastest in *best.  results[0..capacity) (may be NULL) receives
  * every trial; *count (may be NULL) their number.  The winner is recorded for
  * the size class (log2 of cells) of the average grid, and TWS_KERNEL_AUTO
- * picks it from then on in this process.  Allocates and frees its own
- * device memory; never touches existing simulations. */
+ * picks it from then on in this process.  Pass the step count of the runs
+ * being tuned for: a count that is not a multiple of k ends in a partial
+ * launch, which costs a full launch's memory traffic.  Allocates and frees
+ * its own device memory; never touches existing simulations. */
 int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32_t steps,
                  tws_tune_entry* best, tws_tune_entry* results, int32_t capacity,
                  int32_t* count);

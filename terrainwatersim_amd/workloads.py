@@ -96,7 +96,9 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
     grids = batch_grids(n_grids, seed)
     cells = int(sum(s * s for s, _, _ in grids))
     log(f"batched: {n_grids} grids, {cells / 1e6:.1f} M cells; autotuning")
-    best, trials = tws.autotune([(s, s) for s, _, _ in grids], steps=50)
+    # tuned on the step count of the runs it serves: a trial whose steps leave
+    # a partial last launch would misjudge k
+    best, trials = tws.autotune([(s, s) for s, _, _ in grids], steps=steps)
     pool = sum(tws.grid_footprint(s, s) for s, _, _ in grids)
     out = {"grids": n_grids, "cells": cells, "steps_per_run": steps,
            "autotune_best": best, "autotune_trials": trials}

@@ -422,8 +422,14 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ 
     const int levels = min(KH, g.steps - launch_idx * KH);
     if (levels <= 0) return;
     const int parity = (g.parity0 + launch_idx) & 1;
+    // Every other launch walks the grid's tiles backwards, so it starts on the
+    // tiles the launch before wrote last, whose state may still sit in the
+    // memory-side cache.  Tiles of one launch are independent: the order does
+    // not change results.
+    int lt = tile - g.tile_begin;
+    if (parity) lt = g.tiles_x * g.tiles_y - 1 - lt;
     int tx, ty;
-    tile_coords(g, tile - g.tile_begin, tx, ty);
+    tile_coords(g, lt, tx, ty);
     const bool bad = tile_step<CPL, RPW, NW, KH, GEN>(g, tx, ty, levels, parity);
     if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(g.status, 1u);
 }

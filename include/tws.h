@@ -431,6 +431,19 @@ int tws_generate_fbm_terrain(int32_t width, int32_t height, uint64_t seed, int32
                              int32_t base_frequency, float persistence, float height_scale,
                              float* out);
 
+/* The reference's own default scene, bit for bit: the terrain and lake of
+ * Terrain::CreateHeightmapFromNoiseAndResetSim (Terrain.cpp:200-222) with its
+ * NoiseGenerator (NoiseGenerator.cpp:6-97: 4096 white-noise values, 3-D value
+ * noise over octaves 2..10, persistence 0.43, periodic) fed by its Mersenne-
+ * Twister variant (Random.cpp:22-56) seeded with `seed` (Application.cpp:57
+ * uses 231656522).  scene_index n gives the scene the reference builds on its
+ * (n+1)-th call, after n NoiseGenerators have drawn 4096 values each.
+ * terrain and depth (may be NULL) receive resolution^2 floats, row-major
+ * (x + y*resolution, as Terrain.cpp:216-220).  The lake's pow(r^2, 2) is
+ * taken as the correctly rounded square.  Host memory, threaded. */
+int tws_generate_reference_scene(int32_t resolution, float height_scale, uint32_t seed,
+                                 int32_t scene_index, float* terrain, float* depth);
+
 /* The same fBm computed on the device straight into a simulation's terrain
  * plane (bit-identical to tws_generate_fbm_terrain), and a device-side water
  * initialiser: depth `depth` everywhere except rect_depth inside

@@ -12,10 +12,11 @@
 //     to 10 per call and drops the excess time (Terrain.cpp:240-249);
 //   - ApplyRadialWaterBrush wraps the world position into the grid and uses
 //     radius^2 = 32 texels (Terrain.cpp:150-168);
-//   - CreateHeightmapFromNoiseAndResetSim builds terrain (0..300 m) and the
-//     default lake depth max(0, (0.45 - 800 r^4) * 300 - b), with r the
-//     distance from the grid centre in units of the grid side
-//     (Terrain.cpp:200-238); the terrain noise is this library's seeded fBm.
+//   - CreateHeightmapFromNoiseAndResetSim builds the reference's own scene,
+//     bit for bit: terrain (0..300 m) from its NoiseGenerator and PRNG
+//     (seed 231656522 as Application.cpp:57, the n-th call drawing the n-th
+//     noise table) and the default lake depth max(0, (0.45 - 800 r^4) * 300 - b)
+//     (Terrain.cpp:200-238), through tws_generate_reference_scene.
 //   - after a frame with steps the TerrainData mip chain is regenerated into
 //     the target set by SetMipChainTarget (Terrain.cpp:270-276).
 // Error behaviour: the reference's methods return void and report failures
@@ -54,21 +55,13 @@ public:
         last_steps_ = n;
     }
 
-    // Terrain from noise, lake in the middle, zero flux.
+    // Terrain from noise, lake in the middle, zero flux: the reference's scene.
     void CreateHeightmapFromNoiseAndResetSim() {
         const int N = (int)resolution_;
         std::vector<float> b((size_t)N * N), d((size_t)N * N);
-        if (!Check(tws_generate_fbm_terrain(N, N, seed_, 9, 4, 0.43f, height_scale_, b.data())))
+        if (!Check(tws_generate_reference_scene(N, height_scale_, (uint32_t)seed_, scenes_++,
+                                                b.data(), d.data())))
             return;
-        const float m = 1.0f / (float)(N > 1 ? N - 1 : 1);
-        for (int y = 0; y < N; ++y)
-            for (int x = 0; x < N; ++x) {
-                const float dx = x * m - 0.5f, dy = y * m - 0.5f;
-                const float r2 = dx * dx + dy * dy;
-                const float lake = (0.45f - (float)std::pow(r2, 2.0f) * 800.0f) * height_scale_;
-                const size_t i = (size_t)y * N + x;
-                d[i] = std::max(0.0f, lake - b[i]);
-            }
         Check(tws_upload(sim_, b.data(), d.data(), nullptr));
     }
 
@@ -160,6 +153,7 @@ private:
     float world_size_;
     uint64_t seed_;
     float height_scale_ = 300.0f;
+    int32_t scenes_ = 0;  // noise tables drawn so far (one per scene)
     float steps_per_second_ = 60.0f;
     float damping_ = 0.98f;
     float acceleration_ = 10.0f;

@@ -20,7 +20,7 @@ from ._lib import (BOUNDARY_CLOSED, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x
 
 __all__ = [
     "Simulation", "Batch", "Server", "Params", "TwsError", "grid_footprint", "params_physical", "params_reference",
-    "generate_fbm_terrain", "device_count", "lane_shift_probe", "lib", "LIB_PATH", "PROTOTYPES",
+    "generate_fbm_terrain", "generate_reference_scene", "device_count", "lane_shift_probe", "lib", "LIB_PATH", "PROTOTYPES",
     "BOUNDARY_CLOSED", "BOUNDARY_ZERO", "KERNEL_AUTO", "KERNEL_TWOPASS", "KERNEL_TILE_128x128",
     "KERNEL_TILE_128x64", "KERNEL_TILE_256x64", "KERNEL_TILE_64x64", "KERNEL_TILE_64x64_16W",
     "TILE_KERNELS",
@@ -63,6 +63,17 @@ def params_raw(F: float, K: float, Cc: float, boundary=BOUNDARY_CLOSED,
     p.flow_friction, p.water_acceleration, p.cell_area_inv_timescaled = F, K, Cc
     p.boundary, p.ghost_alpha = boundary, ghost_alpha
     return p
+
+
+def generate_reference_scene(resolution: int = 1024, height_scale: float = 300.0,
+                             seed: int = 231656522, scene_index: int = 0):
+    """The reference's own default scene (terrain b, lake depth d), bit for bit:
+    tws_generate_reference_scene.  Returns two (resolution, resolution) arrays."""
+    b = np.empty((resolution, resolution), np.float32)
+    d = np.empty((resolution, resolution), np.float32)
+    check(lib().tws_generate_reference_scene(int(resolution), float(height_scale), int(seed),
+                                             int(scene_index), _ptr(b), _ptr(d)))
+    return b, d
 
 
 def generate_fbm_terrain(width: int, height: int, seed: int, octaves: int = 6,

@@ -154,6 +154,7 @@ PROTOTYPES = {
     "tws_get_launch": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32)]),
     "tws_set_tile_order": (C.c_int, [_vp, _i32]),
     "tws_generate_fbm_terrain": (C.c_int, [_i32, _i32, C.c_uint64, _i32, _i32, C.c_float, C.c_float, _vp]),
+    "tws_generate_reference_scene": (C.c_int, [_i32, C.c_float, C.c_uint32, _i32, _vp, _vp]),
     "tws_init_terrain_fbm": (C.c_int, [_vp, C.c_uint64, _i32, _i32, C.c_float, C.c_float]),
     "tws_init_water": (C.c_int, [_vp, C.c_float, _i32, _i32, _i32, _i32, C.c_float]),
     "tws_batch_init_terrain_fbm": (C.c_int, [_vp, _i32, C.c_uint64, _i32, _i32, C.c_float, C.c_float]),

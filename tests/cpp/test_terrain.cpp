@@ -37,10 +37,13 @@ int main() {
             lake += d[y * N + x];
             if (800.0 * r2 * r2 > 0.4501 && d[y * N + x] > 0.0f) ++outside_wet;
         }
-    // (how much water the lake holds depends on the terrain noise, which is this
-    // library's fBm rather than the reference's PRNG, so only its shape is pinned)
-    EXPECT(lake >= 0.0);
+    EXPECT(lake > 0.0);
     EXPECT(outside_wet == 0);
+    // ... and it is the reference's own scene (seed of Application.cpp:57)
+    std::vector<float> rb(N * N), rd(N * N);
+    EXPECT(tws_generate_reference_scene(N, 300.0f, 231656522u, 0, rb.data(), rd.data()) == TWS_OK);
+    EXPECT(std::memcmp(rb.data(), b.data(), sizeof(float) * N * N) == 0);
+    EXPECT(std::memcmp(rd.data(), d.data(), sizeof(float) * N * N) == 0);
 
     // 51 ms at 60 steps/s: 3 steps (the step length is the float 1/60 held as
     // double, 0.01666666754 s, so exactly 50 ms would give only 2); then a

@@ -294,3 +294,21 @@ def test_perform_simulation_step_clamps_like_reference(gpu):
         for bad in (float("inf"), float("nan"), -1.0):
             with pytest.raises(gpu.TwsError):
                 s.perform_simulation_step(bad, 1.0 / 60.0)
+
+
+def test_reference_default_scene_steps_bitwise(gpu, oracle):
+    """The drop-in default: the reference's own scene (its PRNG and noise), its
+    default constants (60 steps/s, damping 0.98, acceleration 10) and its own
+    out-of-image edge, 30 steps in 3 calls: bit-identical to the restatement."""
+    N = 256
+    b, d = gpu.generate_reference_scene(N)
+    p = gpu.params_reference(boundary=gpu.BOUNDARY_ZERO)
+    with gpu.Simulation(N, N, p) as s:
+        s.upload(b, d)
+        for n in (1, 9, 20):
+            s.step(n)
+        out = s.download(flux=True, velocity=True)
+    de, fe, ve = oracle.step(b, d, None, *p3(p), n=30, boundary=oracle.BOUNDARY_ZERO,
+                             velocity=True)
+    assert_same(out, de, fe, ve)
+    assert out["depth"].sum() > 0

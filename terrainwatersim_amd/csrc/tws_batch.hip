@@ -117,7 +117,12 @@ static int run_launches(tws_batch* b, int launches, int blocks, bool general) {
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return hip_fail(e, "batch: instantiate graph");
-        it = b->graphs.emplace(key, exec).first;
+        try {
+            it = b->graphs.emplace(key, exec).first;
+        } catch (...) {
+            (void)hipGraphExecDestroy(exec);
+            return fail(TWS_ERR_NOMEM, "batch: graph cache");
+        }
         ++b->captures;
     }
     TWS_HIP(hipGraphLaunch(it->second, b->stream), "batch: graph launch");
@@ -140,7 +145,12 @@ int tws_batch_create(int32_t max_grids, int64_t pool_bytes, void* stream, tws_ba
     tws_batch* b = new (std::nothrow) tws_batch();
     if (!b) return fail(TWS_ERR_NOMEM, "tws_batch_create: host allocation failed");
     b->max_grids = max_grids;
-    b->slots.resize((size_t)max_grids);
+    try {
+        b->slots.resize((size_t)max_grids);
+    } catch (...) {
+        delete b;
+        return fail(TWS_ERR_NOMEM, "tws_batch_create: host allocation failed");
+    }
     hipError_t e = hipSuccess;
     if (stream) {
         b->stream = (hipStream_t)stream;

@@ -14,15 +14,14 @@
 
 namespace tws {
 
-static thread_local std::string g_last_error;
+// A fixed buffer: recording an error allocates nothing and cannot throw.
+static thread_local char g_last_error[512];
 
 int fail(int code, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
     va_end(ap);
-    g_last_error = buf;
     return code;
 }
 
@@ -76,7 +75,7 @@ extern "C" {
 
 int tws_abi_version(void) { return TWS_ABI_VERSION; }
 
-const char* tws_last_error(void) { return tws::g_last_error.c_str(); }
+const char* tws_last_error(void) { return tws::g_last_error; }
 
 int tws_device_count(void) {
     int n = 0;

@@ -9,8 +9,6 @@
 
 #include <algorithm>
 #include <cstdint>
-#include <thread>
-#include <vector>
 
 #include "tws.h"
 #include "tws_runtime.hpp"
@@ -151,12 +149,6 @@ extern "C" int tws_generate_reference_scene(int32_t resolution, float height_sca
                 }
             }
     };
-    const int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) {
-        const int y0 = (int)((int64_t)N * t / nt), y1 = (int)((int64_t)N * (t + 1) / nt);
-        if (y0 < y1) th.emplace_back(rows, y0, y1);
-    }
-    for (auto& t : th) t.join();
+    tws::parallel_rows(N, 16u, rows);
     return TWS_OK;
 }

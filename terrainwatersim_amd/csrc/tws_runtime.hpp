@@ -1,7 +1,10 @@
 // tws_runtime.hpp -- host runtime pieces shared by the C-ABI front ends.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+#include <thread>
+#include <vector>
 
 #include "tws.h"
 #include "tws_internal.hpp"
@@ -17,5 +20,34 @@ bool valid_params(const tws_params* p);
 // Launch configuration (kernel variant, steps per launch) for a grid size,
 // from the tuning table in tws_tune.cpp.
 void choose_launch(int width, int height, int* kernel, int* k);
+
+// Runs body(y0, y1) over the rows [0, n) on up to max_threads host threads.
+// Rows whose thread cannot be started run on the calling thread, so no
+// exception leaves it (the C ABI must not throw).
+template <class Body>
+void parallel_rows(int n, unsigned max_threads, Body&& body) noexcept {
+    unsigned nt = std::max(1u, std::min(max_threads, std::thread::hardware_concurrency()));
+    nt = std::min<unsigned>(nt, (unsigned)std::max(n, 1));
+    std::vector<std::thread> pool;
+    try {
+        pool.reserve(nt);
+    } catch (...) {
+        nt = 1;
+    }
+    for (unsigned t = 0; t < nt; ++t) {
+        const int y0 = (int)((int64_t)n * t / nt), y1 = (int)((int64_t)n * (t + 1) / nt);
+        if (y0 >= y1) continue;
+        bool started = false;
+        if (t + 1 < nt) {
+            try {
+                pool.emplace_back(body, y0, y1);
+                started = true;
+            } catch (...) {
+            }
+        }
+        if (!started) body(y0, y1);
+    }
+    for (auto& th : pool) th.join();
+}
 
 }  // namespace tws

@@ -114,6 +114,7 @@ struct tws_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     bool stop = false;
+    bool broken = false;            // the loop stopped on a host allocation failure
     uint64_t next_ticket = 1;
     std::map<uint64_t, ReqPtr> all;
     std::deque<ReqPtr> queue;
@@ -126,6 +127,7 @@ struct tws_server {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     void loop();
+    void fail_all() noexcept;
     void finish(const ReqPtr& q, int state, int error);
     int deliver(const ReqPtr& q);
     void free_inputs(const ReqPtr& q) {
@@ -151,6 +153,28 @@ void tws_server::finish(const ReqPtr& q, int state, int error) {
         ++stats.cancelled;
     }
     free_inputs(q);
+    cv_done.notify_all();
+}
+
+// The loop's host bookkeeping threw (out of memory): every request not yet
+// finished fails, wherever the loop had it, and the server takes no more.
+void tws_server::fail_all() noexcept {
+    (void)hipStreamSynchronize(io);
+    (void)tws_batch_sync(batch);
+    std::lock_guard<std::mutex> lk(mu);
+    broken = true;
+    try {
+        for (auto& kv : all) {
+            const ReqPtr& q = kv.second;
+            if (q->st.state != TWS_REQ_QUEUED && q->st.state != TWS_REQ_RUNNING) continue;
+            if (q->gid >= 0) (void)tws_batch_remove(batch, q->gid);
+            finish(q, TWS_REQ_FAILED, TWS_ERR_NOMEM);
+        }
+        queue.clear();
+        resident.clear();
+        finishing.clear();
+    } catch (...) {
+    }
     cv_done.notify_all();
 }
 
@@ -354,7 +378,21 @@ int tws_server_create(const tws_server_config* config, tws_server** out) {
     tws_batch_stats bs{};
     tws_batch_stats_get(s->batch, &bs);
     s->stats.pool_bytes = bs.pool_bytes;
-    s->th = std::thread([s] { s->loop(); });
+    try {
+        s->th = std::thread([s] {
+            try {
+                s->loop();
+            } catch (...) {
+                s->fail_all();
+            }
+        });
+    } catch (...) {
+        (void)hipEventDestroy(s->cleared);
+        (void)hipStreamDestroy(s->io);
+        tws_batch_destroy(s->batch);
+        delete s;
+        return fail(TWS_ERR_NOMEM, "tws_server_create: cannot start the server thread");
+    }
     *out = s;
     return TWS_OK;
 }
@@ -370,7 +408,12 @@ int tws_server_submit(tws_server* s, const tws_request* r, uint64_t* ticket) {
     if (fp > s->stats.pool_bytes)
         return fail(TWS_ERR_CAPACITY, "tws_server_submit: footprint %lld exceeds the pool",
                     (long long)fp);
-    auto q = std::make_shared<Req>();
+    ReqPtr q;
+    try {
+        q = std::make_shared<Req>();
+    } catch (...) {
+        return fail(TWS_ERR_NOMEM, "tws_server_submit: host allocation failed");
+    }
     q->r = *r;
     q->footprint = fp;
     q->steps_left = r->steps;
@@ -388,15 +431,31 @@ int tws_server_submit(tws_server* s, const tws_request* r, uint64_t* ticket) {
         std::memcpy(q->flux_in, r->flux4, sizeof(float) * 4 * cells);
     }
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->broken) {
+        s->free_inputs(q);
+        return fail(TWS_ERR_STATE, "tws_server_submit: the server stopped after a host "
+                    "allocation failure");
+    }
     if ((int64_t)s->queue.size() >= s->cfg.max_queue) {
         s->free_inputs(q);
         return fail(TWS_ERR_CAPACITY, "tws_server_submit: queue full");
     }
-    q->ticket = s->next_ticket++;
+    q->ticket = s->next_ticket;
     q->st.state = TWS_REQ_QUEUED;
     q->st.t_submit = s->now();
-    s->all[q->ticket] = q;
-    s->queue.push_back(q);
+    try {
+        s->all[q->ticket] = q;
+        try {
+            s->queue.push_back(q);
+        } catch (...) {
+            s->all.erase(q->ticket);
+            throw;
+        }
+    } catch (...) {
+        s->free_inputs(q);
+        return fail(TWS_ERR_NOMEM, "tws_server_submit: host allocation failed");
+    }
+    ++s->next_ticket;
     ++s->stats.submitted;
     s->stats.queued = (int64_t)s->queue.size();
     *ticket = q->ticket;

@@ -188,7 +188,12 @@ int tws_step_profiled(tws_sim* s, int32_t n, float* launch_ms, int32_t capacity,
                       int32_t* launches) {
     if (!s || !launch_ms || capacity <= 0 || !launches)
         return fail(TWS_ERR_INVALID, "tws_step_profiled: invalid argument");
-    std::vector<hipEvent_t> ev(2 * (size_t)capacity, nullptr);
+    std::vector<hipEvent_t> ev;
+    try {
+        ev.assign(2 * (size_t)capacity, nullptr);
+    } catch (...) {
+        return fail(TWS_ERR_NOMEM, "tws_step_profiled: host allocation failed");
+    }
     for (auto& e : ev) {
         hipError_t r = hipEventCreate(&e);
         if (r != hipSuccess) {

@@ -105,8 +105,13 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
         rc = tws_batch_add(b, widths[i], heights[i], &p, &id);
         if (rc != TWS_OK) break;
         const size_t c = (size_t)widths[i] * heights[i];
-        terrain.resize(c);
-        depth.assign(c, 0.5f);
+        try {
+            terrain.resize(c);
+            depth.assign(c, 0.5f);
+        } catch (...) {
+            rc = fail(TWS_ERR_NOMEM, "tws_autotune: host allocation failed");
+            break;
+        }
         rc = tws_generate_fbm_terrain(widths[i], heights[i], 1000u + (uint64_t)i, 6, 4, 0.43f,
                                       100.0f, terrain.data());
         if (rc == TWS_OK) rc = tws_batch_upload(b, id, terrain.data(), depth.data(), nullptr);
@@ -146,7 +151,11 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
     {
         // record for the size class of the typical grid of the set
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[size_class(cells / n)] = {top.kernel, top.steps_per_launch};
+        try {
+            g_tuned[size_class(cells / n)] = {top.kernel, top.steps_per_launch};
+        } catch (...) {
+            return fail(TWS_ERR_NOMEM, "tws_autotune: recording the result");
+        }
     }
     return TWS_OK;
 }

@@ -96,6 +96,39 @@ k_mip_next(const float4* __restrict__ src, int ws, int hs, float4* __restrict__ 
                                            bilerp(s00.w, s01.w, s10.w, s11.w, tx, ty));
 }
 
+// The small end of the chain in one work group: every level from one of at
+// most kTailTexels texels down to 1x1, level after level, each read back
+// from the chain buffer that this work group wrote before the barrier.  The
+// per-texel arithmetic is k_mip_next's; the levels cost one launch instead of
+// one each (they are bound by launch latency, not by bytes).
+constexpr int kTailThreads = 1024;
+constexpr int kTailTexels = 64 * 64;
+
+__global__ void __launch_bounds__(kTailThreads)
+k_mip_tail(const float4* __restrict__ src, int ws, int hs, float4* __restrict__ dst) {
+    while (ws > 1 || hs > 1) {
+        const int wd = ws / 2 > 0 ? ws / 2 : 1, hd = hs / 2 > 0 ? hs / 2 : 1;
+        for (int k = threadIdx.x; k < wd * hd; k += kTailThreads) {
+            const int j = k / wd, i = k - j * wd;
+            int x0, x1, y0, y1;
+            float tx, ty;
+            mip_axis(i, ws, wd, x0, x1, tx);
+            mip_axis(j, hs, hd, y0, y1, ty);
+            const float4 s00 = src[y0 * ws + x0], s01 = src[y0 * ws + x1];
+            const float4 s10 = src[y1 * ws + x0], s11 = src[y1 * ws + x1];
+            dst[k] = make_float4(bilerp(s00.x, s01.x, s10.x, s11.x, tx, ty),
+                                 bilerp(s00.y, s01.y, s10.y, s11.y, tx, ty),
+                                 bilerp(s00.z, s01.z, s10.z, s11.z, tx, ty),
+                                 bilerp(s00.w, s01.w, s10.w, s11.w, tx, ty));
+        }
+        __syncthreads();  // this level is complete and visible to the whole group
+        src = dst;
+        dst += wd * hd;
+        ws = wd;
+        hs = hd;
+    }
+}
+
 dim3 mip_grid(int wd, int hd) { return dim3((wd + kMipBX - 1) / kMipBX, (hd + kMipBY - 1) / kMipBY); }
 
 }  // namespace
@@ -129,6 +162,14 @@ int grid_mips(GridState& g, float gb_fill, float* dst, size_t capacity_floats, h
     int ws = g.W, hs = g.H;
     for (int l = 1; l < levels; ++l) {
         const int wd = g.W >> l > 0 ? g.W >> l : 1, hd = g.H >> l > 0 ? g.H >> l : 1;
+        if (l > 1 && (int64_t)wd * hd <= kTailTexels) {
+            // levels l .. levels-1 in one launch (their sizes follow the same
+            // halving rule, so the tail writes them where the loop would)
+            hipLaunchKernelGGL(k_mip_tail, dim3(1), dim3(kTailThreads), 0, s, prev, ws, hs, out);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return hip_fail(e, "mips: launch");
+            return TWS_OK;
+        }
         if (l == 1)
             hipLaunchKernelGGL(k_mip_first, mip_grid(wd, hd), dim3(kMipBX * kMipBY), 0, s, gd,
                                g.parity, gb_fill, out, wd, hd);

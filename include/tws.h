@@ -234,7 +234,9 @@ int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launc
 /* Order in which the step kernel walks a grid's tiles: groups of
  * `group_columns` tile columns, row by row inside a group, so the tiles in
  * flight together form a compact block that shares its halos in L2
- * (0 = row-major).  Results do not depend on it. */
+ * (0 = row-major).  tws_create picks the measured best for the grid's default
+ * launch (16, or 20 for 128x64 tiles at 5 steps per launch).  Results do not
+ * depend on it. */
 int tws_set_tile_order(tws_sim* sim, int32_t group_columns);
 
 /* Host-side float -> IEEE half conversion rounding toward zero, the FlowMap

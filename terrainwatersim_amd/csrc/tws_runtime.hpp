@@ -20,6 +20,8 @@ bool valid_params(const tws_params* p);
 // Launch configuration (kernel variant, steps per launch) for a grid size,
 // from the tuning table in tws_tune.cpp.
 void choose_launch(int width, int height, int* kernel, int* k);
+// Tile order (columns per group) for a grid's default launch.
+int default_tile_group(int width, int height, int kernel, int k);
 
 // Runs body(y0, y1) over the rows [0, n) on up to max_threads host threads.
 // Rows whose thread cannot be started run on the calling thread, so no

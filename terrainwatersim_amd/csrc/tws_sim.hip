@@ -119,6 +119,7 @@ int tws_create(int32_t width, int32_t height, const tws_params* params, void* st
     }
     if (rc != TWS_OK) { tws_destroy(s); return rc; }
     choose_launch(width, height, &s->kernel, &s->k);
+    s->g.tile_group = default_tile_group(width, height, s->kernel, s->k);
     *out = s;
     return TWS_OK;
 }

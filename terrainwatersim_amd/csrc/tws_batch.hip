@@ -34,11 +34,10 @@ struct Slot {
 };
 
 struct GraphKey {
-    int kernel, k, launches, blocks;
-    bool general;
+    int kernel, k, launches, blocks, mode;
     bool operator<(const GraphKey& o) const {
-        return std::tie(kernel, k, launches, blocks, general) <
-               std::tie(o.kernel, o.k, o.launches, o.blocks, o.general);
+        return std::tie(kernel, k, launches, blocks, mode) <
+               std::tie(o.kernel, o.k, o.launches, o.blocks, o.mode);
     }
 };
 
@@ -92,14 +91,14 @@ static void drop_graphs(tws_batch* b) {
 }
 
 // Launches (or replays) the tile kernels of one run.
-static int run_launches(tws_batch* b, int launches, int blocks, bool general) {
+static int run_launches(tws_batch* b, int launches, int blocks, int mode) {
     if (!b->use_graphs) {
         for (int j = 0; j < launches; ++j)
-            TWS_HIP(launch_tile_batch(b->kernel, b->k, general, b->table_dev, b->hdr_dev, blocks,
+            TWS_HIP(launch_tile_batch(b->kernel, b->k, mode, b->table_dev, b->hdr_dev, blocks,
                                       j, b->stream), "batch: tile kernel");
         return TWS_OK;
     }
-    const GraphKey key{b->kernel, b->k, launches, blocks, general};
+    const GraphKey key{b->kernel, b->k, launches, blocks, mode};
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
         if (b->graphs.size() >= 32) drop_graphs(b);
@@ -108,7 +107,7 @@ static int run_launches(tws_batch* b, int launches, int blocks, bool general) {
                 "batch: begin capture");
         hipError_t e = hipSuccess;
         for (int j = 0; j < launches && e == hipSuccess; ++j)
-            e = launch_tile_batch(b->kernel, b->k, general, b->table_dev, b->hdr_dev, blocks, j,
+            e = launch_tile_batch(b->kernel, b->k, mode, b->table_dev, b->hdr_dev, blocks, j,
                                   b->stream);
         hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
         if (e != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return hip_fail(e, "batch: capture"); }
@@ -309,7 +308,7 @@ int tws_batch_run(tws_batch* b) {
     // the pinned table may still be in flight from the previous run
     TWS_HIP(hipEventSynchronize(b->table_copied), "tws_batch_run: wait table");
     int n = 0, tiles = 0, max_steps = 0;
-    bool general = false;
+    int mode = kModePlain;
     for (int i = 0; i < b->max_grids; ++i) {
         Slot& s = b->slots[i];
         if (!s.used || s.steps <= 0) continue;
@@ -318,7 +317,7 @@ int tws_batch_run(tws_batch* b) {
             return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
         g.tile_begin = tiles;
         g.steps = s.steps;
-        general |= needs_general_kernel(g);
+        mode = std::max(mode, kernel_mode(g));
         tiles += g.tiles_x * g.tiles_y;
         max_steps = std::max(max_steps, s.steps);
         b->table_host[n++] = g;
@@ -334,7 +333,7 @@ int tws_batch_run(tws_batch* b) {
     const int launches = (max_steps + b->k - 1) / b->k;
     // block bound rounded up so small changes of the tile count reuse a graph
     const int blocks = (tiles + 511) / 512 * 512;
-    int rc = run_launches(b, launches, blocks, general);
+    int rc = run_launches(b, launches, blocks, mode);
     if (rc != TWS_OK) return rc;
     for (int i = 0; i < b->max_grids; ++i) {
         Slot& s = b->slots[i];

@@ -73,13 +73,17 @@ struct TileShape {
 };
 bool tile_shape(int kernel, int k, TileShape* out);
 
+// Step-kernel modes: what a launch compiles in.  Plain: closed walls without
+// rain.  Rain: closed walls, rain on some grids.  General: any edge mode.
+// Each mode runs every grid a lower mode can.
+constexpr int kModePlain = 0, kModeRain = 1, kModeGeneral = 2;
+
 // Launchers (stream-ordered, return hipError_t).
 // Batch launch: hdr_dev = {number of grids, number of tiles}; `blocks` is an
-// upper bound of the tile count (extra blocks exit).  `general` must be true
-// if any grid of the table has rain or an edge other than closed walls
-// (needs_general_kernel); the other kernel compiles those paths out.
-bool needs_general_kernel(const GridDesc& g);
-hipError_t launch_tile_batch(int kernel, int k, bool general, const GridDesc* grids_dev,
+// upper bound of the tile count (extra blocks exit).  `mode` must be at least
+// kernel_mode(g) of every grid of the table.
+int kernel_mode(const GridDesc& g);
+hipError_t launch_tile_batch(int kernel, int k, int mode, const GridDesc* grids_dev,
                              const int* hdr_dev, int blocks, int launch_idx, hipStream_t s);
 hipError_t launch_twopass(const GridDesc* g_dev, const GridDesc& g_host, int parity,
                           hipStream_t s);

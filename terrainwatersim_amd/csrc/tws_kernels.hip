@@ -159,13 +159,18 @@ __device__ __forceinline__ void tile_coords(const GridDesc& g, int lt, int& tx, 
 
 // One tile: load its EXW x EXH block of state set `parity`, advance it
 // `levels` steps, store the valid interior into the other set.  Returns
-// whether this lane stored a non-finite value.  GEN = false compiles out
-// the rain source and the reference-edge masking (for closed walls without
-// rain: no per-cell domain masks, so no scalar spills and 111 instead of
-// 128 VGPRs at 128x64, measured +2.6% at k = 4, +7% at k = 6).
-template <int CPL, int RPW, int NW, int KH, bool GEN>
+// whether this lane stored a non-finite value.  MODE (kModePlain,
+// kModeRain, kModeGeneral) compiles in only what the launch's grids use:
+// the plain kernel has neither the rain source nor the reference-edge
+// masking (no per-cell domain masks, so no scalar spills and 111 instead of
+// 128 VGPRs at 128x64: +2.6% at k = 4, +7% at k = 6); the rain kernel has
+// the rain but not the edge masking (6 instead of 23 scalar spills: +0.6% to
+// +1.8% on closed-wall grids with rain).
+template <int CPL, int RPW, int NW, int KH, int MODE>
 __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int levels,
                                           int parity) {
+    constexpr bool GEN = MODE != kModePlain;   // rain source compiled in
+    constexpr bool EDGE = MODE == kModeGeneral;  // reference-edge masking compiled in
     using G = TileGeom<CPL, RPW, NW, KH>;
     constexpr int EXW = G::EXW, EXH = G::EXH;
     __shared__ float s_b[EXH * EXW];
@@ -232,7 +237,7 @@ __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int
             }
         }
     }
-    const bool need_mask = GEN && g.boundary == TWS_BOUNDARY_ZERO &&
+    const bool need_mask = EDGE && g.boundary == TWS_BOUNDARY_ZERO &&
                            (ox < 0 || oy < 0 || ox + EXW > W || oy + EXH > H);
     __syncthreads();  // s_b complete
 
@@ -390,10 +395,10 @@ __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int
 // the tile count is gridDim.x.  Batch: hdr = {number of grids, number of
 // tiles} and grids[] (ordered by tile_begin) live in device memory, so a
 // captured launch stays valid while grids come and go; blocks past the tile
-// count exit at once.  GEN = false only for launches in which every grid has
-// closed walls and no rain (chosen on the host from the descriptors).
+// count exit at once.  MODE is the highest kernel_mode of the launch's grids
+// (chosen on the host from the descriptors).
 // At least 4 waves per SIMD (at most 128 VGPRs): 2 work groups of 512 per CU.
-template <int CPL, int RPW, int NW, int KH, bool GEN>
+template <int CPL, int RPW, int NW, int KH, int MODE>
 __global__ void __launch_bounds__(NW * kWave, 4)
 k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ hdr,
        int launch_idx) {
@@ -430,7 +435,7 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ 
     if (parity) lt = g.tiles_x * g.tiles_y - 1 - lt;
     int tx, ty;
     tile_coords(g, lt, tx, ty);
-    const bool bad = tile_step<CPL, RPW, NW, KH, GEN>(g, tx, ty, levels, parity);
+    const bool bad = tile_step<CPL, RPW, NW, KH, MODE>(g, tx, ty, levels, parity);
     if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(g.status, 1u);
 }
 
@@ -672,31 +677,34 @@ __global__ void __launch_bounds__(64) k_lane_shift_probe(int* out) {
 
 static dim3 cell_grid(const GridDesc& g) { return dim3((g.W + 15) / 16, (g.H + 15) / 16); }
 
-template <int CPL, int RPW, int NW, bool GEN>
+template <int CPL, int RPW, int NW, int MODE>
 static hipError_t launch_tile_kg(int k, const GridDesc* grids_dev, const int* hdr,
                                  const GridDesc& g1, int blocks, int launch_idx, hipStream_t s) {
     const dim3 grid(blocks), block(NW * kWave);
     switch (k) {
-        case 1: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 1, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
-        case 2: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 2, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
-        case 3: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 3, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
-        case 4: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 4, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
-        case 5: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 5, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
-        case 6: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 6, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
-        case 8: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 8, GEN>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 1: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 1, MODE>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 2: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 2, MODE>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 3: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 3, MODE>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 4: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 4, MODE>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 5: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 5, MODE>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 6: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 6, MODE>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
+        case 8: hipLaunchKernelGGL((k_tile<CPL, RPW, NW, 8, MODE>), grid, block, 0, s, g1, grids_dev, hdr, launch_idx); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <int CPL, int RPW, int NW>
-static hipError_t launch_tile_k(int k, bool gen, const GridDesc* grids_dev, const int* hdr,
+static hipError_t launch_tile_k(int k, int mode, const GridDesc* grids_dev, const int* hdr,
                                 const GridDesc* single, int blocks, int launch_idx,
                                 hipStream_t s) {
     GridDesc g1{};
     if (single) g1 = *single;
-    return gen ? launch_tile_kg<CPL, RPW, NW, true>(k, grids_dev, hdr, g1, blocks, launch_idx, s)
-               : launch_tile_kg<CPL, RPW, NW, false>(k, grids_dev, hdr, g1, blocks, launch_idx, s);
+    switch (mode) {
+        case kModePlain: return launch_tile_kg<CPL, RPW, NW, kModePlain>(k, grids_dev, hdr, g1, blocks, launch_idx, s);
+        case kModeRain: return launch_tile_kg<CPL, RPW, NW, kModeRain>(k, grids_dev, hdr, g1, blocks, launch_idx, s);
+        default: return launch_tile_kg<CPL, RPW, NW, kModeGeneral>(k, grids_dev, hdr, g1, blocks, launch_idx, s);
+    }
 }
 
 template <int CPL, int RPW, int NW>
@@ -726,32 +734,33 @@ bool tile_shape(int kernel, int k, TileShape* out) {
     }
 }
 
-static hipError_t launch_tile_any(int kernel, int k, bool gen, const GridDesc* grids_dev,
+static hipError_t launch_tile_any(int kernel, int k, int mode, const GridDesc* grids_dev,
                                   const int* hdr, const GridDesc* single, int blocks,
                                   int launch_idx, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
     switch (kernel) {
-        case TWS_KERNEL_TILE_128x128: return launch_tile_k<2, 8, 16>(k, gen, grids_dev, hdr, single, blocks, launch_idx, s);
-        case TWS_KERNEL_TILE_128x64: return launch_tile_k<2, 8, 8>(k, gen, grids_dev, hdr, single, blocks, launch_idx, s);
-        case TWS_KERNEL_TILE_256x64: return launch_tile_k<4, 4, 16>(k, gen, grids_dev, hdr, single, blocks, launch_idx, s);
-        case TWS_KERNEL_TILE_64x64: return launch_tile_k<1, 8, 8>(k, gen, grids_dev, hdr, single, blocks, launch_idx, s);
-        case TWS_KERNEL_TILE_64x64_16W: return launch_tile_k<1, 4, 16>(k, gen, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_128x128: return launch_tile_k<2, 8, 16>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_128x64: return launch_tile_k<2, 8, 8>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_256x64: return launch_tile_k<4, 4, 16>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_64x64: return launch_tile_k<1, 8, 8>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_64x64_16W: return launch_tile_k<1, 4, 16>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-bool needs_general_kernel(const GridDesc& g) {
-    return g.rain_on != 0 || g.boundary != TWS_BOUNDARY_CLOSED;
+int kernel_mode(const GridDesc& g) {
+    if (g.boundary != TWS_BOUNDARY_CLOSED) return kModeGeneral;
+    return g.rain_on != 0 ? kModeRain : kModePlain;
 }
 
-hipError_t launch_tile_batch(int kernel, int k, bool general, const GridDesc* grids_dev,
+hipError_t launch_tile_batch(int kernel, int k, int mode, const GridDesc* grids_dev,
                              const int* hdr_dev, int blocks, int launch_idx, hipStream_t s) {
-    return launch_tile_any(kernel, k, general, grids_dev, hdr_dev, nullptr, blocks, launch_idx, s);
+    return launch_tile_any(kernel, k, mode, grids_dev, hdr_dev, nullptr, blocks, launch_idx, s);
 }
 
 hipError_t launch_tile_single(int kernel, int k, const GridDesc& g, int launch_idx,
                               hipStream_t s) {
-    return launch_tile_any(kernel, k, needs_general_kernel(g), nullptr, nullptr, &g,
+    return launch_tile_any(kernel, k, kernel_mode(g), nullptr, nullptr, &g,
                            g.tiles_x * g.tiles_y, launch_idx, s);
 }
 

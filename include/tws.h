@@ -462,6 +462,19 @@ int tws_batch_init_terrain_fbm(tws_batch* batch, int32_t grid_id, uint64_t seed,
 int tws_batch_init_water(tws_batch* batch, int32_t grid_id, float depth, int32_t x0, int32_t y0,
                          int32_t x1, int32_t y1, float rect_depth);
 
+/* The reference's default scene (as tws_generate_reference_scene, the same
+ * bits) computed on the device into a square grid: terrain, lake depth and
+ * zero flux, replacing the host loop and the uploads of
+ * Terrain::CreateHeightmapFromNoiseAndResetSim (Terrain.cpp:209-234).  Only
+ * the 4096-value noise table is drawn on the host; it is staged through the
+ * grid's idle flux set, so grids must be at least 16 x 16.  Synchronises the
+ * handle's stream.  TWS_ERR_INVALID for a non-square grid, a NaN
+ * height_scale or scene_index < 0. */
+int tws_init_reference_scene(tws_sim* sim, float height_scale, uint32_t seed,
+                             int32_t scene_index);
+int tws_batch_init_reference_scene(tws_batch* batch, int32_t grid_id, float height_scale,
+                                   uint32_t seed, int32_t scene_index);
+
 /* ---- diagnostics ---------------------------------------------------------- */
 
 /* Lane-layout check of the step kernel's cross-lane moves: lane i of one

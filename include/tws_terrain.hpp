@@ -55,9 +55,14 @@ public:
         last_steps_ = n;
     }
 
-    // Terrain from noise, lake in the middle, zero flux: the reference's scene.
+    // Terrain from noise, lake in the middle, zero flux: the reference's scene,
+    // computed on the device (below 16 x 16 on the host, then uploaded).
     void CreateHeightmapFromNoiseAndResetSim() {
         const int N = (int)resolution_;
+        if (N >= 16) {
+            Check(tws_init_reference_scene(sim_, height_scale_, (uint32_t)seed_, scenes_++));
+            return;
+        }
         std::vector<float> b((size_t)N * N), d((size_t)N * N);
         if (!Check(tws_generate_reference_scene(N, height_scale_, (uint32_t)seed_, scenes_++,
                                                 b.data(), d.data())))

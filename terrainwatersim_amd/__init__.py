@@ -246,6 +246,13 @@ class Simulation:
         x0, y0, x1, y1 = rect
         check(lib().tws_init_water(self._h, depth, x0, y0, x1, y1, rect_depth))
 
+    def init_reference_scene(self, height_scale: float = 300.0, seed: int = 231656522,
+                             scene_index: int = 0):
+        """The reference's default scene computed on the device (the bits of
+        generate_reference_scene); square grids of at least 16 x 16."""
+        check(lib().tws_init_reference_scene(self._h, float(height_scale), int(seed),
+                                             int(scene_index)))
+
     def set_tile_order(self, group_columns: int):
         check(lib().tws_set_tile_order(self._h, int(group_columns)))
 
@@ -366,6 +373,11 @@ class Batch:
     def init_water(self, gid, depth: float = 0.0, rect=(0, 0, 0, 0), rect_depth: float = 0.0):
         x0, y0, x1, y1 = rect
         check(lib().tws_batch_init_water(self._h, int(gid), depth, x0, y0, x1, y1, rect_depth))
+
+    def init_reference_scene(self, gid, height_scale: float = 300.0, seed: int = 231656522,
+                             scene_index: int = 0):
+        check(lib().tws_batch_init_reference_scene(self._h, int(gid), float(height_scale),
+                                                   int(seed), int(scene_index)))
 
     def set_rain(self, gid, cx, cy, intensity, radius_sq):
         check(lib().tws_batch_set_rain(self._h, int(gid), cx, cy, intensity, radius_sq))

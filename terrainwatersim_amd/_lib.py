@@ -159,6 +159,8 @@ PROTOTYPES = {
     "tws_init_water": (C.c_int, [_vp, C.c_float, _i32, _i32, _i32, _i32, C.c_float]),
     "tws_batch_init_terrain_fbm": (C.c_int, [_vp, _i32, C.c_uint64, _i32, _i32, C.c_float, C.c_float]),
     "tws_batch_init_water": (C.c_int, [_vp, _i32, C.c_float, _i32, _i32, _i32, _i32, C.c_float]),
+    "tws_init_reference_scene": (C.c_int, [_vp, C.c_float, C.c_uint32, _i32]),
+    "tws_batch_init_reference_scene": (C.c_int, [_vp, _i32, C.c_float, C.c_uint32, _i32]),
     "tws_debug_lane_shift_probe": (C.c_int, [C.POINTER(_i32)]),
     "tws_float_to_half_rtz": (C.c_int, [_vp, _vp, C.c_int64]),
     "tws_stream_copy_bench": (C.c_int, [C.c_int64, _i32, C.POINTER(C.c_double)]),

@@ -270,6 +270,13 @@ int tws_batch_init_water(tws_batch* b, int32_t id, float depth, int32_t x0, int3
     return grid_init_water(s->g, depth, x0, y0, x1, y1, rect_depth, b->stream);
 }
 
+int tws_batch_init_reference_scene(tws_batch* b, int32_t id, float height_scale, uint32_t seed,
+                                   int32_t scene_index) {
+    Slot* s = slot_of(b, id);
+    if (!s) return fail(TWS_ERR_INVALID, "tws_batch_init_reference_scene: unknown grid %d", id);
+    return grid_init_reference_scene(s->g, height_scale, seed, scene_index, b->stream);
+}
+
 int tws_batch_set_params(tws_batch* b, int32_t id, const tws_params* p) {
     Slot* s = slot_of(b, id);
     if (!s) return fail(TWS_ERR_INVALID, "tws_batch_set_params: unknown grid %d", id);

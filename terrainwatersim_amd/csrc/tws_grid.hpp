@@ -59,6 +59,10 @@ int grid_init_terrain_fbm(GridState& g, uint64_t seed, int octaves, int base, fl
                           float scale, hipStream_t s);
 int grid_init_water(GridState& g, float depth, int x0, int y0, int x1, int y1, float rect_depth,
                     hipStream_t s);
+// The reference's default scene (tws_refscene.cpp): b, the lake depth and
+// zero flux of a square grid; synchronises the stream.
+int grid_init_reference_scene(GridState& g, float height_scale, uint32_t seed,
+                              int32_t scene_index, hipStream_t s);
 
 // Terrain mip chain (tws_mips.hip): level count and floats of levels 1..,
 // and the chain written into device memory `dst` on stream s.

@@ -344,6 +344,11 @@ int tws_init_water(tws_sim* s, float depth, int32_t x0, int32_t y0, int32_t x1, 
     return grid_init_water(s->g, depth, x0, y0, x1, y1, rect_depth, s->stream);
 }
 
+int tws_init_reference_scene(tws_sim* s, float height_scale, uint32_t seed, int32_t scene_index) {
+    if (!s) return fail(TWS_ERR_INVALID, "tws_init_reference_scene: NULL handle");
+    return grid_init_reference_scene(s->g, height_scale, seed, scene_index, s->stream);
+}
+
 int tws_set_tile_order(tws_sim* s, int32_t group_columns) {
     if (!s || group_columns < 0) return fail(TWS_ERR_INVALID, "tws_set_tile_order: bad argument");
     s->g.tile_group = group_columns;

@@ -25,6 +25,9 @@ int main() {
     // initial state, captured through the reference layout
     std::vector<float> t0, f0;
     terrain.ReadReferenceTextures(&t0, &f0, nullptr);
+    int flux_nonzero = 0;  // the scene starts at rest (Terrain.cpp:231-233)
+    for (float v : f0) flux_nonzero += v != 0.0f;
+    EXPECT(flux_nonzero == 0);
     std::vector<float> b(N * N), d(N * N);
     for (int i = 0; i < N * N; ++i) { b[i] = t0[i * 4 + 0]; d[i] = t0[i * 4 + 3]; }
     // default lake: water only inside 800 r^4 < 0.45 (r from the grid centre)

@@ -43,3 +43,84 @@ def test_batch_device_init(gpu):
         batch.init_water(g, 0.0, (10, 10, 20, 20), 5.0)
         d = batch.download(g, flux=False)["depth"]
     assert d.sum() == 5.0 * 100
+
+
+# -- the reference's default scene on the device (tws_init_reference_scene) --
+
+import tws_refscene as R  # noqa: E402  (oracle restatement)
+
+R_SEED = 231656522  # Application.cpp:57
+
+
+@pytest.mark.parametrize("N,scene_index,height_scale", [(16, 0, 300.0), (100, 0, 300.0),
+                                                        (257, 1, 123.5), (513, 0, 300.0)])
+def test_device_reference_scene_equals_restatement(gpu, N, scene_index, height_scale):
+    b0, d0 = R.scene(N, height_scale, R_SEED, scene_index)
+    with gpu.Simulation(N, N) as s:
+        s.init_reference_scene(height_scale, R_SEED, scene_index)
+        t = s.download_reference_layout()["terrain_rgba"]
+        f = s.download(flux=True)["flux"]
+    assert np.array_equal(t[..., 0], b0)
+    assert np.array_equal(t[..., 3], d0)
+    assert not f.any()
+
+
+def test_device_reference_scene_equals_host_at_4096(gpu):
+    N = 4096
+    b0, d0 = gpu.generate_reference_scene(N)
+    with gpu.Simulation(N, N) as s:
+        s.init_reference_scene()
+        t = s.download_reference_layout()["terrain_rgba"]
+    assert np.array_equal(t[..., 0], b0)
+    assert np.array_equal(t[..., 3], d0)
+
+
+def test_device_reference_scene_resets_a_stepped_grid(gpu, oracle):
+    """After steps that left the state in either ping-pong set, the scene
+    replaces b, d and f; stepping it equals stepping the uploaded host scene."""
+    N = 128
+    b, d = gpu.generate_reference_scene(N)
+    p = gpu.params_reference(boundary=gpu.BOUNDARY_ZERO)
+    for pre in (1, 2):
+        with gpu.Simulation(N, N, p) as s:
+            s.set_launch(gpu.KERNEL_TILE_128x64, 4)
+            s.init_terrain_fbm(9)
+            s.init_water(1.0)
+            for _ in range(pre):
+                s.step(4)
+            s.init_reference_scene()
+            out0 = s.download(flux=True)
+            s.step(12)
+            out = s.download(flux=True, velocity=True)
+        assert np.array_equal(out0["depth"], d) and not out0["flux"].any()
+        de, fe, ve = oracle.step(b, d, None, p.flow_friction, p.water_acceleration,
+                                 p.cell_area_inv_timescaled, n=12,
+                                 boundary=oracle.BOUNDARY_ZERO, velocity=True)
+        assert np.array_equal(out["depth"], de) and np.array_equal(out["flux"], fe)
+        assert np.array_equal(out["velocity"], ve)
+
+
+def test_device_reference_scene_rejects_bad_grids(gpu):
+    with gpu.Simulation(64, 32) as s:
+        with pytest.raises(gpu.TwsError):
+            s.init_reference_scene()
+    with gpu.Simulation(8, 8) as s:
+        with pytest.raises(gpu.TwsError):
+            s.init_reference_scene()
+    with gpu.Simulation(32, 32) as s:
+        with pytest.raises(gpu.TwsError):
+            s.init_reference_scene(scene_index=-1)
+        with pytest.raises(gpu.TwsError):
+            s.init_reference_scene(height_scale=float("nan"))
+
+
+def test_batch_device_reference_scene(gpu):
+    b0, d0 = gpu.generate_reference_scene(64, 200.0, 7, 2)
+    with gpu.Batch(4, 1 << 28) as batch:
+        g = batch.add(64, 64)
+        other = batch.add(64, 48)
+        batch.init_reference_scene(g, 200.0, 7, 2)
+        with pytest.raises(gpu.TwsError):
+            batch.init_reference_scene(other)
+        d = batch.download(g)
+    assert np.array_equal(d["depth"], d0) and not d["flux"].any()

@@ -225,7 +225,10 @@ int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 #define TWS_KERNEL_TILE_64x64_16W 6 /* 1024 threads, 1 column x 4 rows each:
                                        the 64x64 tile with half the serial
                                        work per wave and level               */
-#define TWS_KERNEL_COUNT 7
+#define TWS_KERNEL_TILE_64x48_16W 7 /* 1024 threads, 1 column x 3 rows each:
+                                       64x48 tiles, so grids up to 384^2
+                                       spread over more CUs                  */
+#define TWS_KERNEL_COUNT 8
 
 /* Selects a kernel variant and the number of steps per launch (1, 2, 3, 4,
  * 5, 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */

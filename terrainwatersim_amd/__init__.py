@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import (BOUNDARY_CLOSED, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x64,
                    KERNEL_TILE_128x128, KERNEL_TILE_256x64, KERNEL_TILE_64x64,
-                   KERNEL_TILE_64x64_16W, KERNEL_TWOPASS, LIB_PATH,
+                   KERNEL_TILE_64x64_16W, KERNEL_TILE_64x48_16W, KERNEL_TWOPASS, LIB_PATH,
                    PROTOTYPES, REQ_CANCELLED, REQ_DONE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING,
                    TILE_KERNELS, TWS_ERR_CAPACITY, BatchStats, DevicePlanes, Params, Request,
                    RequestStatus, ServerConfig, ServerStats, TuneEntry, TwsError, check, lib)
@@ -23,6 +23,7 @@ __all__ = [
     "generate_fbm_terrain", "generate_reference_scene", "device_count", "lane_shift_probe", "lib", "LIB_PATH", "PROTOTYPES",
     "BOUNDARY_CLOSED", "BOUNDARY_ZERO", "KERNEL_AUTO", "KERNEL_TWOPASS", "KERNEL_TILE_128x128",
     "KERNEL_TILE_128x64", "KERNEL_TILE_256x64", "KERNEL_TILE_64x64", "KERNEL_TILE_64x64_16W",
+    "KERNEL_TILE_64x48_16W",
     "TILE_KERNELS",
 ]
 

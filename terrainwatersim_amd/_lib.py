@@ -31,8 +31,9 @@ KERNEL_TILE_128x64 = 3
 KERNEL_TILE_256x64 = 4
 KERNEL_TILE_64x64 = 5
 KERNEL_TILE_64x64_16W = 6
+KERNEL_TILE_64x48_16W = 7
 TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64, KERNEL_TILE_64x64,
-                KERNEL_TILE_64x64_16W)
+                KERNEL_TILE_64x64_16W, KERNEL_TILE_64x48_16W)
 
 
 class TwsError(RuntimeError):

@@ -730,6 +730,7 @@ bool tile_shape(int kernel, int k, TileShape* out) {
         case TWS_KERNEL_TILE_256x64: shape_of<4, 4, 16>(k, out); return true;
         case TWS_KERNEL_TILE_64x64: shape_of<1, 8, 8>(k, out); return true;
         case TWS_KERNEL_TILE_64x64_16W: shape_of<1, 4, 16>(k, out); return true;
+        case TWS_KERNEL_TILE_64x48_16W: shape_of<1, 3, 16>(k, out); return true;
         default: return false;
     }
 }
@@ -744,6 +745,7 @@ static hipError_t launch_tile_any(int kernel, int k, int mode, const GridDesc* g
         case TWS_KERNEL_TILE_256x64: return launch_tile_k<4, 4, 16>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
         case TWS_KERNEL_TILE_64x64: return launch_tile_k<1, 8, 8>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
         case TWS_KERNEL_TILE_64x64_16W: return launch_tile_k<1, 4, 16>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
+        case TWS_KERNEL_TILE_64x48_16W: return launch_tile_k<1, 3, 16>(k, mode, grids_dev, hdr, single, blocks, launch_idx, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -46,10 +46,17 @@ void choose_launch(int width, int height, int* kernel, int* k) {
     }
     // MI355X sweep (cell-updates/s; 0.5 m sheet on fBm terrain, with and without
     // a rain disc; stale halo rows skipped in the 64-wide tiles):
+    //   <= 384^2   latency-bound, few tiles: 64x48 tiles over 16 waves of 3
+    //              rows put the grid on more CUs; ms per 1000 steps with rain:
+    //              256^2 1.70 (64x48, k = 8; 128 tiles) vs 2.10 (64x64, k = 8);
+    //              128^2 1.65 vs 2.07; 384^2 2.12 (64x48, k = 6; 160 tiles) vs
+    //              2.23 (64x64, k = 8) vs 3.11 (64x48, k = 8: 288 tiles, more
+    //              than one per CU)
     //   <= 512^2   latency-bound: 64x64 tiles spread the grid over more CUs, and
     //              16 waves of 4 rows (instead of 8 of 8) halve each wave's
     //              serial work per level: 512^2 1.16e11 (64x64 16 waves, k = 8)
-    //              vs 8.7e10 (8 waves, k = 6) vs 5.1e10 (128x64, k = 5)
+    //              vs 8.7e10 (8 waves, k = 6) vs 5.1e10 (128x64, k = 5);
+    //              64x48 tiles: 2.93 ms per 1000 steps at best vs 2.38
     //   <= 2048^2  64x64 tiles, 8 waves, k = 4: 2048^2 2.98e11 with rain
     //              (3.31e11 without) vs 2.67e11 (2.91e11) for 128x64, k = 5;
     //              1024^2 1.88e11 with rain vs 1.72e11
@@ -57,7 +64,13 @@ void choose_launch(int width, int height, int* kernel, int* k) {
     //              3.08e11 (3.42e11) for 64x64, k = 4
     //   larger     128x64, k = 4: the 16384^2 dam-break 4.30e11 vs 4.10e11 at
     //              k = 5 (on an all-wet 16384^2 sheet k = 5 leads by 4%).
-    if (cells <= 512 * 512) {
+    if (cells <= 256 * 256) {
+        *kernel = TWS_KERNEL_TILE_64x48_16W;
+        *k = 8;
+    } else if (cells <= 384 * 384) {
+        *kernel = TWS_KERNEL_TILE_64x48_16W;
+        *k = 6;
+    } else if (cells <= 512 * 512) {
         *kernel = TWS_KERNEL_TILE_64x64_16W;
         *k = 8;
     } else if (cells <= 2048 * 2048) {
@@ -133,7 +146,7 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
     }
     static const int kernels[] = {TWS_KERNEL_TILE_128x64, TWS_KERNEL_TILE_128x128,
                                   TWS_KERNEL_TILE_256x64, TWS_KERNEL_TILE_64x64,
-                                  TWS_KERNEL_TILE_64x64_16W};
+                                  TWS_KERNEL_TILE_64x64_16W, TWS_KERNEL_TILE_64x48_16W};
     static const int ks[] = {2, 3, 4, 5, 6, 8};
     tws_tune_entry top{0, 0, 0.0};
     int m = 0;

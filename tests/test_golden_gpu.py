@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 MANIFEST = json.loads((GOLDEN / "manifest.json").read_text())
 CASES = sorted(MANIFEST["cases"])
-LAUNCHES = [(0, 0), (1, 1), (2, 1), (2, 8), (3, 4), (3, 5), (4, 2), (5, 3), (5, 8), (6, 2), (6, 8)]
+LAUNCHES = [(0, 0), (1, 1), (2, 1), (2, 8), (3, 4), (3, 5), (4, 2), (5, 3), (5, 8), (6, 2), (6, 8),
+            (7, 3), (7, 6), (7, 8)]
 
 
 @pytest.mark.parametrize("name", CASES)

@@ -56,7 +56,8 @@ def test_lane_shift_layout(gpu):
 
 
 KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 5), (3, 8), (4, 1), (4, 2), (4, 4),
-           (4, 5), (4, 8), (5, 1), (5, 4), (5, 5), (5, 6), (5, 8), (6, 1), (6, 3), (6, 5), (6, 8)]
+           (4, 5), (4, 8), (5, 1), (5, 4), (5, 5), (5, 6), (5, 8), (6, 1), (6, 3), (6, 5), (6, 8),
+           (7, 1), (7, 2), (7, 4), (7, 6), (7, 8)]
 
 
 @pytest.mark.parametrize("kernel,k", KERNELS)
@@ -102,7 +103,8 @@ def test_twopass_bitwise(gpu, oracle):
     assert_same(out, de, fe, ve)
 
 
-@pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 2), (4, 8), (3, 5), (5, 4), (6, 8)])
+@pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 2), (4, 8), (3, 5), (5, 4), (6, 8),
+                                      (7, 6), (7, 8)])
 def test_zero_boundary_bitwise(gpu, oracle, kernel, k):
     b, d, f = random_state(21, 130, 259)
     p = gpu.params_physical(**CFG_PARAMS, boundary=gpu.BOUNDARY_ZERO)
@@ -117,7 +119,7 @@ def test_ragged_and_tiny_grids(gpu, oracle, shape):
     H, W = shape
     b, d, f = random_state(H * 1000 + W, H, W)
     p = gpu.params_physical(**CFG_PARAMS)
-    for kernel, k in ((2, 4), (4, 8), (1, 1), (3, 5), (5, 6), (6, 8)):
+    for kernel, k in ((2, 4), (4, 8), (1, 1), (3, 5), (5, 6), (6, 8), (7, 8)):
         out = run_gpu(gpu, b, d, f, 6, kernel, k, p)
         de, fe, ve = oracle.step(b, d, f, *p3(p), n=6, velocity=True)
         assert_same(out, de, fe, ve)

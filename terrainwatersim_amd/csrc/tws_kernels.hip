@@ -239,21 +239,34 @@ __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int
     }
     const bool need_mask = EDGE && g.boundary == TWS_BOUNDARY_ZERO &&
                            (ox < 0 || oy < 0 || ox + EXW > W || oy + EXH > H);
+    // The brush amounts do not change from level to level: computed once, in
+    // registers across the loop (recomputing them per level measured 13%
+    // slower at 256^2, where waves under the disc are on the critical path),
+    // and only by waves under the disc.  Left to itself the compiler hoisted
+    // the divisions out of the loop into every wave.
+    float wet[RPW][CPL];
+    if (rain) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+                wet[r][c] = brush_amount(g.rain_cx, g.rain_cy, g.rain_I, g.rain_R2, x0 + c,
+                                         oy + ly + r);
+    } else {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) wet[r][c] = 0.0f;
+    }
     __syncthreads();  // s_b complete
 
     for (int lev = 0; lev < levels; ++lev) {
         if (rain) {
-            // The brush amounts do not change from level to level; the compiler
-            // keeps them in registers across the loop (one VGPR spill at 128x64,
-            // but waves under the disc are on the critical path of latency-bound
-            // grids: recomputing them per level measured 13% slower at 256^2).
 #pragma unroll
             for (int r = 0; r < RPW; ++r)
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
-                    if (inside & (1u << (r * CPL + c)))
-                        d[r][c] = d[r][c] + brush_amount(g.rain_cx, g.rain_cy, g.rain_I,
-                                                         g.rain_R2, x0 + c, oy + ly + r);
+                    if (inside & (1u << (r * CPL + c))) d[r][c] = d[r][c] + wet[r][c];
         }
 
         // ---- flux phase: flowUpdate.comp:34-62 ------------------------------

@@ -33,6 +33,36 @@ static int size_class(int64_t cells) {
     return c;
 }
 
+// 512^2 < cells <= 832^2: a CU holding two work groups of a 16-wave tile
+// runs both at half speed, so the best launch follows the number of rounds
+// of tiles on 256 CUs.  Each band below keeps one round count for its launch
+// over the whole band (square grids).  Same-run sweep, 1e9 cell-updates/s with a rain disc
+// (without), against 64x64 at k = 4:
+//   576^2  64x48_16W k=4: 108 (109) vs 81 (83)      1 round
+//   640^2  64x48_16W k=6: 102 (126) vs 95 (98)      2 rounds
+//   704^2  64x64_16W k=8: 119 (142) vs 111 (113)    2 rounds
+//   768^2  64x48_16W k=5: 133 (148) vs 126 (128)    2 rounds
+//   832^2  64x48_16W k=4: 142 (149) vs 120 (123)    2 rounds
+// Above 832^2 64x64 at k = 4 leads again (896^2: 142 vs at most 138 with rain).
+static void band_launch(int64_t cells, int* kernel, int* k) {
+    if (cells <= 576 * 576) {
+        *kernel = TWS_KERNEL_TILE_64x48_16W;
+        *k = 4;
+    } else if (cells <= 640 * 640) {
+        *kernel = TWS_KERNEL_TILE_64x48_16W;
+        *k = 6;
+    } else if (cells <= 704 * 704) {
+        *kernel = TWS_KERNEL_TILE_64x64_16W;
+        *k = 8;
+    } else if (cells <= 768 * 768) {
+        *kernel = TWS_KERNEL_TILE_64x48_16W;
+        *k = 5;
+    } else {
+        *kernel = TWS_KERNEL_TILE_64x48_16W;
+        *k = 4;
+    }
+}
+
 void choose_launch(int width, int height, int* kernel, int* k) {
     const int64_t cells = (int64_t)width * height;
     {
@@ -57,6 +87,7 @@ void choose_launch(int width, int height, int* kernel, int* k) {
     //              serial work per level: 512^2 1.16e11 (64x64 16 waves, k = 8)
     //              vs 8.7e10 (8 waves, k = 6) vs 5.1e10 (128x64, k = 5);
     //              64x48 tiles: 2.93 ms per 1000 steps at best vs 2.38
+    //   <= 832^2   bands by tile rounds (band_launch)
     //   <= 2048^2  64x64 tiles, 8 waves, k = 4: 2048^2 2.98e11 with rain
     //              (3.31e11 without) vs 2.67e11 (2.91e11) for 128x64, k = 5;
     //              1024^2 1.88e11 with rain vs 1.72e11
@@ -73,6 +104,8 @@ void choose_launch(int width, int height, int* kernel, int* k) {
     } else if (cells <= 512 * 512) {
         *kernel = TWS_KERNEL_TILE_64x64_16W;
         *k = 8;
+    } else if (cells <= 832 * 832) {
+        band_launch(cells, kernel, k);
     } else if (cells <= 2048 * 2048) {
         *kernel = TWS_KERNEL_TILE_64x64;
         *k = 4;

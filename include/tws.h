@@ -50,13 +50,34 @@ extern "C" {
 #define TWS_BOUNDARY_ZERO 1     /* the reference shaders' own edge: texels
                                    outside read as r=g=b=0, a=ghost_alpha and
                                    water drains into them
-                                   (flowUpdate.comp:14-18, flowApply.comp:16-20) */
+                                   (flowUpdate.comp:14-18, flowApply.comp:16-20).
+                                   Every cell is updated: identical to the
+                                   shaders when W and H are multiples of 16
+                                   (of 32 with rain or the brush), see below */
+#define TWS_BOUNDARY_REFERENCE_DISPATCH 2
+                                /* ZERO plus the reference's dispatch shapes,
+                                   so every size steps exactly as the shaders
+                                   do: the two passes run W/16 x H/16 work
+                                   groups of 16 x 16 cells (Terrain.cpp:258,
+                                   264) and the brush W/32 x H/32 of 32 x 32
+                                   (Terrain.cpp:167), integer divisions.
+                                   Cells in columns >= 16*(W/16) or rows >=
+                                   16*(H/16) keep their depth and flux forever
+                                   (neighbours still read them) and have no
+                                   directed flow (0); the brush and the rain
+                                   reach only columns < 32*(W/32) and rows
+                                   < 32*(H/32).  Where W and H are multiples of
+                                   32 this is ZERO.  Other sizes step on the
+                                   two-pass kernels, single grids only (a batch
+                                   refuses them). */
 
 /* Per-step constants of the SimulationParameters block
  * (simulationCommon.glsl:1-13) plus the boundary mode.  F, K and C must be
- * finite and >= 0, as the reference derives them (Terrain.cpp:175-198):
- * closed walls are exact only while a head difference toward a wall cannot
- * push water into it (K >= 0) for terrain and depths below ~1e28. */
+ * finite and >= 0, as the reference derives them (Terrain.cpp:175-198), and
+ * with closed walls K <= 1e8 (the wall's ghost cell stands 1e30 high and its
+ * head difference times K must stay finite).  Closed walls are then exact
+ * for any state: a wall-ward flux uploaded into an edge cell is dropped when
+ * the step loads it. */
 typedef struct tws_params {
     float flow_friction;            /* F = FlowFriction_perStep             */
     float water_acceleration;       /* K = WaterAcceleration_perStep        */
@@ -208,6 +229,7 @@ int tws_check_finite(tws_sim* sim);
 int tws_sync(tws_sim* sim);
 int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 
+
 /* ---- launch configuration ------------------------------------------------ */
 
 /* Kernel variants.  TWOPASS mirrors the reference's structure (one flux
@@ -228,7 +250,14 @@ int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 #define TWS_KERNEL_TILE_64x48_16W 7 /* 1024 threads, 1 column x 3 rows each:
                                        64x48 tiles, so grids up to 384^2
                                        spread over more CUs                  */
-#define TWS_KERNEL_COUNT 8
+#define TWS_KERNEL_STREAM_128 8     /* streaming strips: each wavefront sweeps
+                                       a 128-column strip segment down once
+                                       per launch with all k levels in a
+                                       register window, so only the x-halo is
+                                       recomputed; single grids with closed
+                                       walls (others step on the default
+                                       tile kernel); k = 2, 3, 4, 5, 6 or 8 */
+#define TWS_KERNEL_COUNT 9
 
 /* Selects a kernel variant and the number of steps per launch (1, 2, 3, 4,
  * 5, 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */
@@ -321,16 +350,21 @@ typedef struct tws_tune_entry {
  * fBm terrain under a 0.5 m water sheet, a rain disc of radius 16 at each
  * grid's centre), `steps` steps per trial after an untimed run of as many, and
  * returns the fastest in *best.  results[0..capacity) (may be NULL) receives
- * every trial; *count (may be NULL) their number.  The winner is recorded for
- * the size class (log2 of cells) of the average grid, and TWS_KERNEL_AUTO
- * picks it from then on in this process.  Pass the step count of the runs
- * being tuned for: a count that is not a multiple of k ends in a partial
- * launch, which costs a full launch's memory traffic.  Allocates and frees
- * its own device memory; never touches existing simulations. */
+ * every trial; *count (may be NULL) their number.  Records nothing: the
+ * launch TWS_KERNEL_AUTO picks changes only through tws_tune_record.  Pass
+ * the step count of the runs being tuned for: a count that is not a multiple
+ * of k ends in a partial launch, which costs a full launch's memory traffic.
+ * Allocates and frees its own device memory; never touches existing
+ * simulations. */
 int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32_t steps,
                  tws_tune_entry* best, tws_tune_entry* results, int32_t capacity,
                  int32_t* count);
-/* Drops every recorded autotuning result. */
+/* Enters (kernel, steps_per_launch) for the size class (log2 of the cell
+ * count) of a width x height grid: TWS_KERNEL_AUTO (tws_create,
+ * tws_set_launch) picks it for grids of that class from then on in this
+ * process.  TWS_ERR_INVALID for an unknown tile variant or k. */
+int tws_tune_record(int32_t width, int32_t height, int32_t kernel, int32_t steps_per_launch);
+/* Drops every recorded launch: TWS_KERNEL_AUTO follows the built-in table. */
 int tws_tune_forget(void);
 
 /* ---- serving: a request loop over a batch --------------------------------- */
@@ -487,9 +521,13 @@ int tws_batch_init_reference_scene(tws_batch* batch, int32_t grid_id, float heig
 int tws_debug_lane_shift_probe(int32_t* out128);
 
 /* Device-to-device STREAM copy of `bytes` (read + write), `iterations`
- * kernel launches on the default stream; *gbps receives (2 * bytes *
- * iterations) / time.  The measured HBM peak next to the 8 TB/s spec. */
-int tws_stream_copy_bench(int64_t bytes, int32_t iterations, double* gbps);
+ * kernel launches on the default stream, in launch shape `shape` (0 <= shape
+ * < tws_stream_copy_shapes(): loads in flight per lane, non-temporal hints,
+ * work groups per CU; shape 0 keeps one load per lane in flight), or the
+ * fastest of all shapes for shape = -1.  *gbps receives (2 * bytes *
+ * iterations) / time: the measured HBM ceiling next to the 8 TB/s spec. */
+int tws_stream_copy_bench(int64_t bytes, int32_t iterations, int32_t shape, double* gbps);
+int tws_stream_copy_shapes(void);
 
 const char* tws_last_error(void);
 int tws_abi_version(void);

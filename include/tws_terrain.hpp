@@ -19,6 +19,12 @@
 //     (Terrain.cpp:200-238), through tws_generate_reference_scene.
 //   - after a frame with steps the TerrainData mip chain is regenerated into
 //     the target set by SetMipChainTarget (Terrain.cpp:270-276).
+//   - the edge: by default TWS_BOUNDARY_REFERENCE_DISPATCH, i.e. the shaders'
+//     own out-of-image reads (water drains off the edges) with their
+//     dispatch shapes, so a resolution that is not a multiple of 32 leaves the
+//     same tail untouched as Terrain.cpp:167,258,264 do.  At multiples of 32
+//     that is TWS_BOUNDARY_ZERO and runs the fused kernels; TWS_BOUNDARY_CLOSED
+//     (walls) can be asked for.
 // Error behaviour: the reference's methods return void and report failures
 // through its log; here they return void as well, print the library's
 // message to stderr and keep it for Ok()/LastError().
@@ -38,8 +44,10 @@ namespace tws {
 class Terrain {
 public:
     explicit Terrain(uint32_t grid_resolution = 1024, float grid_world_size = 1024.0f,
-                     void* hip_stream = nullptr, uint64_t seed = 231656522u)
-        : resolution_(grid_resolution), world_size_(grid_world_size), seed_(seed) {
+                     void* hip_stream = nullptr, uint64_t seed = 231656522u,
+                     int32_t boundary = TWS_BOUNDARY_REFERENCE_DISPATCH)
+        : resolution_(grid_resolution), world_size_(grid_world_size), seed_(seed),
+          boundary_(boundary) {
         DeriveParams();
         Check(tws_create((int32_t)resolution_, (int32_t)resolution_, &params_, hip_stream, &sim_));
         if (sim_) CreateHeightmapFromNoiseAndResetSim();
@@ -145,7 +153,7 @@ private:
         const float ss = steps_per_second_;
         step_length_ = (double)(1.0f / ss);  // Terrain.cpp:177
         Check(tws_params_reference(ss, damping_, acceleration_, world_size_, resolution_,
-                                   TWS_BOUNDARY_CLOSED, &params_));
+                                   boundary_, &params_));
     }
     bool Check(int rc) {
         if (rc == TWS_OK) return true;
@@ -157,6 +165,7 @@ private:
     uint32_t resolution_;
     float world_size_;
     uint64_t seed_;
+    int32_t boundary_;
     float height_scale_ = 300.0f;
     int32_t scenes_ = 0;  // noise tables drawn so far (one per scene)
     float steps_per_second_ = 60.0f;

@@ -316,8 +316,11 @@ int glref_run(int W, int H, const float* terrain_rgba, const float* flow_rgba, f
               float C, const float* rain, int nsteps, float* out_terrain, float* out_flow,
               uint16_t* out_flowmap, double* seconds) {
     if (!g_ctx) { g_err = "glref_init not called"; return -1; }
-    if (W <= 0 || H <= 0 || W % 32 || H % 32) {
-        g_err = "grid must be a positive multiple of 32 (the reference dispatch shapes)";
+    // Any size: the dispatches below divide it with truncation, as
+    // Terrain.cpp:167,258,264 do, so a tail of columns and rows that no work
+    // group covers stays as uploaded.
+    if (W <= 0 || H <= 0) {
+        g_err = "grid must be at least 1 x 1";
         return -1;
     }
     GLuint terrain = make_tex(GL_RGBA32F, W, H);

@@ -41,6 +41,7 @@
 
 #define TWS_ORACLE_BOUNDARY_CLOSED 0
 #define TWS_ORACLE_BOUNDARY_ZERO 1
+#define TWS_ORACLE_BOUNDARY_DISPATCH 2
 
 /* ---------------------------------------------------------------------------
  * Constants: Terrain::SetSimulationStepsPerSecond / SetFlowDamping /

@@ -24,6 +24,7 @@ LIB = HERE / "_build" / "libtws_oracle.so"
 
 BOUNDARY_CLOSED = 0
 BOUNDARY_ZERO = 1
+BOUNDARY_DISPATCH = 2   # ZERO plus the reference's truncating dispatch shapes
 
 _lib = None
 

@@ -18,14 +18,14 @@ typedef ORACLE_REAL ORACLE_CAT(oracle_real, ORACLE_SFX);
  * the whole grid).  The brush parameters are float uniforms, so they enter
  * the double instantiation as the same float values.
  */
-void ORACLE_FN(oracle_brush)(int W, int H, ORACLE_REAL* d, float cx, float cy,
-                             float intensity, float radius_sq)
+static void ORACLE_FN(brush_rect)(int W, int bw, int bh, ORACLE_REAL* d, float cx, float cy,
+                                  float intensity, float radius_sq)
 {
     const ORACLE_REAL one = (ORACLE_REAL)1;
     const ORACLE_REAL zero = (ORACLE_REAL)0;
 #pragma omp parallel for schedule(static)
-    for (int y = 0; y < H; ++y) {
-        for (int x = 0; x < W; ++x) {
+    for (int y = 0; y < bh; ++y) {
+        for (int x = 0; x < bw; ++x) {
             ORACLE_REAL tx = (ORACLE_REAL)cx - (ORACLE_REAL)x;
             ORACLE_REAL ty = (ORACLE_REAL)cy - (ORACLE_REAL)y;
             ORACLE_REAL dist = tx * tx + ty * ty;
@@ -35,6 +35,12 @@ void ORACLE_FN(oracle_brush)(int W, int H, ORACLE_REAL* d, float cx, float cy,
             d[(int64_t)y * W + x] += w * (ORACLE_REAL)intensity;
         }
     }
+}
+
+void ORACLE_FN(oracle_brush)(int W, int H, ORACLE_REAL* d, float cx, float cy,
+                             float intensity, float radius_sq)
+{
+    ORACLE_FN(brush_rect)(W, W, H, d, cx, cy, intensity, radius_sq);
 }
 
 /*
@@ -51,6 +57,14 @@ void ORACLE_FN(oracle_brush)(int W, int H, ORACLE_REAL* d, float cx, float cy,
  *              r=g=b=0 and a=ghost_alpha, both in TerrainData and in Flow
  *              (flowUpdate.comp:18 and flowApply.comp:20 load texel -1 and W
  *              for the edge work groups); ghosts never change.
+ *   DISPATCH -- ZERO, plus the reference's dispatch shapes: both passes run
+ *              W/16 x H/16 work groups of 16 x 16 texels (Terrain.cpp:258,264)
+ *              and the brush W/32 x H/32 groups of 32 x 32 (Terrain.cpp:167),
+ *              integer divisions, so texels in columns >= 16*(W/16) or rows
+ *              >= 16*(H/16) keep their depth and flux (and are never given a
+ *              FlowMap value: v stays as passed in), while their neighbours
+ *              read them as ordinary texels; the brush reaches only columns
+ *              < 32*(W/32) and rows < 32*(H/32).
  * Returns 0, or -1 if the scratch allocation fails.
  */
 int ORACLE_FN(oracle_step)(int W, int H, const ORACLE_REAL* b, ORACLE_REAL* d, ORACLE_REAL* f,
@@ -64,15 +78,22 @@ int ORACLE_FN(oracle_step)(int W, int H, const ORACLE_REAL* b, ORACLE_REAL* d, O
     ORACLE_REAL* fn = (ORACLE_REAL*)malloc(sizeof(ORACLE_REAL) * (size_t)(n * 4));
     if (!fn) return -1;
     const int closed = (boundary == TWS_ORACLE_BOUNDARY_CLOSED);
+    const int dispatch = (boundary == TWS_ORACLE_BOUNDARY_DISPATCH);
+    const int uw = dispatch ? W / 16 * 16 : W, uh = dispatch ? H / 16 * 16 : H;
+    const int bw = dispatch ? W / 32 * 32 : W, bh = dispatch ? H / 32 * 32 : H;
 
     for (int step = 0; step < nsteps; ++step) {
-        if (rain) ORACLE_FN(oracle_brush)(W, H, d, rain[0], rain[1], rain[2], rain[3]);
+        if (rain) ORACLE_FN(brush_rect)(W, bw, bh, d, rain[0], rain[1], rain[2], rain[3]);
 
         /* ---- flux pass, flowUpdate.comp:29-62 ---------------------------- */
 #pragma omp parallel for schedule(static)
         for (int y = 0; y < H; ++y) {
             for (int x = 0; x < W; ++x) {
                 const int64_t i = (int64_t)y * W + x;
+                if (x >= uw || y >= uh) {  /* no work group covers this texel */
+                    for (int c = 0; c < 4; ++c) fn[i * 4 + c] = f[i * 4 + c];
+                    continue;
+                }
                 const int in_x1 = x + 1 < W, in_x0 = x > 0, in_y1 = y + 1 < H, in_y0 = y > 0;
                 ORACLE_REAL own = d[i] + b[i];                                   /* :34 */
                 ORACLE_REAL hx1 = in_x1 ? d[i + 1] + b[i + 1] : ghost_alpha + zero; /* :37 */
@@ -109,6 +130,7 @@ int ORACLE_FN(oracle_step)(int W, int H, const ORACLE_REAL* b, ORACLE_REAL* d, O
         for (int y = 0; y < H; ++y) {
             for (int x = 0; x < W; ++x) {
                 const int64_t i = (int64_t)y * W + x;
+                if (x >= uw || y >= uh) continue;  /* no work group covers this texel */
                 const ORACLE_REAL* o = fn + i * 4;
                 ORACLE_REAL gz = closed ? zero : ghost_alpha;  /* ghost Flow.w */
                 ORACLE_REAL fx1 = x + 1 < W ? fn[(i + 1) * 4 + 1] : zero;        /* :32 X1.y */

@@ -11,9 +11,9 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BOUNDARY_CLOSED, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x64,
+from ._lib import (BOUNDARY_CLOSED, BOUNDARY_REFERENCE_DISPATCH, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x64,
                    KERNEL_TILE_128x128, KERNEL_TILE_256x64, KERNEL_TILE_64x64,
-                   KERNEL_TILE_64x64_16W, KERNEL_TILE_64x48_16W, KERNEL_TWOPASS, LIB_PATH,
+                   KERNEL_TILE_64x64_16W, KERNEL_TILE_64x48_16W, KERNEL_STREAM_128, KERNEL_TWOPASS, LIB_PATH,
                    PROTOTYPES, REQ_CANCELLED, REQ_DONE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING,
                    TILE_KERNELS, TWS_ERR_CAPACITY, BatchStats, DevicePlanes, Params, Request,
                    RequestStatus, ServerConfig, ServerStats, TuneEntry, TwsError, check, lib)
@@ -21,9 +21,9 @@ from ._lib import (BOUNDARY_CLOSED, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x
 __all__ = [
     "Simulation", "Batch", "Server", "Params", "TwsError", "grid_footprint", "params_physical", "params_reference",
     "generate_fbm_terrain", "generate_reference_scene", "device_count", "lane_shift_probe", "lib", "LIB_PATH", "PROTOTYPES",
-    "BOUNDARY_CLOSED", "BOUNDARY_ZERO", "KERNEL_AUTO", "KERNEL_TWOPASS", "KERNEL_TILE_128x128",
+    "BOUNDARY_CLOSED", "BOUNDARY_ZERO", "BOUNDARY_REFERENCE_DISPATCH", "KERNEL_AUTO", "KERNEL_TWOPASS", "KERNEL_TILE_128x128",
     "KERNEL_TILE_128x64", "KERNEL_TILE_256x64", "KERNEL_TILE_64x64", "KERNEL_TILE_64x64_16W",
-    "KERNEL_TILE_64x48_16W",
+    "KERNEL_TILE_64x48_16W", "KERNEL_STREAM_128",
     "TILE_KERNELS",
 ]
 
@@ -111,11 +111,17 @@ def mip_chain_size(width: int, height: int):
     return n.value, f.value
 
 
-def stream_copy_gbps(nbytes: int = 1 << 30, iterations: int = 20) -> float:
-    """Measured device-to-device copy bandwidth (read + write bytes / s, GB/s)."""
+def stream_copy_gbps(nbytes: int = 1 << 30, iterations: int = 20, shape: int = -1) -> float:
+    """Measured device-to-device copy bandwidth (read + write bytes / s, GB/s)
+    in one launch shape, or the best of all shapes (shape = -1)."""
     g = C.c_double()
-    check(lib().tws_stream_copy_bench(int(nbytes), int(iterations), C.byref(g)))
+    check(lib().tws_stream_copy_bench(int(nbytes), int(iterations), int(shape), C.byref(g)))
     return g.value
+
+
+def stream_copy_shapes() -> int:
+    """Number of launch shapes of the copy kernel."""
+    return int(lib().tws_stream_copy_shapes())
 
 
 class Simulation:
@@ -278,9 +284,15 @@ class Simulation:
         return p
 
 
+def tune_record(width: int, height: int, kernel: int, steps_per_launch: int):
+    """Makes TWS_KERNEL_AUTO pick (kernel, k) for grids of this size class."""
+    check(lib().tws_tune_record(int(width), int(height), int(kernel), int(steps_per_launch)))
+
+
 def autotune(shapes, steps: int = 100):
     """Times every (variant, k) candidate on scratch grids of `shapes`
-    [(height, width), ...]; returns (best, all trials) as dicts."""
+    [(height, width), ...]; returns (best, all trials) as dicts.  Records
+    nothing (tune_record does)."""
     hs = np.ascontiguousarray([h for h, w in shapes], np.int32)
     ws = np.ascontiguousarray([w for h, w in shapes], np.int32)
     best = TuneEntry()

@@ -23,6 +23,7 @@ TWS_ERR_CAPACITY = -6
 
 BOUNDARY_CLOSED = 0
 BOUNDARY_ZERO = 1
+BOUNDARY_REFERENCE_DISPATCH = 2
 
 KERNEL_AUTO = 0
 KERNEL_TWOPASS = 1
@@ -32,6 +33,7 @@ KERNEL_TILE_256x64 = 4
 KERNEL_TILE_64x64 = 5
 KERNEL_TILE_64x64_16W = 6
 KERNEL_TILE_64x48_16W = 7
+KERNEL_STREAM_128 = 8
 TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64, KERNEL_TILE_64x64,
                 KERNEL_TILE_64x64_16W, KERNEL_TILE_64x48_16W)
 
@@ -164,7 +166,8 @@ PROTOTYPES = {
     "tws_batch_init_reference_scene": (C.c_int, [_vp, _i32, C.c_float, C.c_uint32, _i32]),
     "tws_debug_lane_shift_probe": (C.c_int, [C.POINTER(_i32)]),
     "tws_float_to_half_rtz": (C.c_int, [_vp, _vp, C.c_int64]),
-    "tws_stream_copy_bench": (C.c_int, [C.c_int64, _i32, C.POINTER(C.c_double)]),
+    "tws_stream_copy_bench": (C.c_int, [C.c_int64, _i32, _i32, C.POINTER(C.c_double)]),
+    "tws_stream_copy_shapes": (C.c_int, []),
     "tws_grid_footprint": (C.c_int64, [_i32, _i32]),
     "tws_batch_create": (C.c_int, [_i32, C.c_int64, _vp, C.POINTER(_vp)]),
     "tws_batch_destroy": (C.c_int, [_vp]),
@@ -186,6 +189,7 @@ PROTOTYPES = {
     "tws_batch_sync": (C.c_int, [_vp]),
     "tws_autotune": (C.c_int, [_vp, _vp, _i32, _i32, C.POINTER(TuneEntry), C.POINTER(TuneEntry), _i32, C.POINTER(_i32)]),
     "tws_tune_forget": (C.c_int, []),
+    "tws_tune_record": (C.c_int, [_i32, _i32, _i32, _i32]),
     "tws_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "tws_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
     "tws_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(RequestStatus)]),

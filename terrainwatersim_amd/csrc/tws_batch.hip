@@ -201,6 +201,9 @@ int tws_batch_add(tws_batch* b, int32_t width, int32_t height, const tws_params*
     if (width <= 0 || height <= 0 || width > (1 << 20) || height > (1 << 20))
         return fail(TWS_ERR_INVALID, "tws_batch_add: invalid size %d x %d", width, height);
     if (!valid_params(params)) return fail(TWS_ERR_INVALID, "tws_batch_add: invalid params");
+    if (!batchable(width, height, *params))
+        return fail(TWS_ERR_INVALID, "tws_batch_add: a reference-dispatch grid of %d x %d (not "
+                    "multiples of 32) steps on the single-grid path only", width, height);
     int id = -1;
     for (int i = 0; i < b->max_grids; ++i)
         if (!b->slots[i].used) { id = i; break; }
@@ -281,6 +284,9 @@ int tws_batch_set_params(tws_batch* b, int32_t id, const tws_params* p) {
     Slot* s = slot_of(b, id);
     if (!s) return fail(TWS_ERR_INVALID, "tws_batch_set_params: unknown grid %d", id);
     if (!valid_params(p)) return fail(TWS_ERR_INVALID, "tws_batch_set_params: invalid params");
+    if (!batchable(s->g.W, s->g.H, *p))
+        return fail(TWS_ERR_INVALID, "tws_batch_set_params: a reference-dispatch grid of %d x %d "
+                    "(not multiples of 32) steps on the single-grid path only", s->g.W, s->g.H);
     s->g.params = *p;
     return TWS_OK;
 }

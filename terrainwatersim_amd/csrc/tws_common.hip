@@ -3,9 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "tws.h"
@@ -35,12 +37,15 @@ bool valid_params(const tws_params* p) {
     if (!std::isfinite(p->flow_friction) || !std::isfinite(p->water_acceleration) ||
         !std::isfinite(p->cell_area_inv_timescaled) || !std::isfinite(p->ghost_alpha))
         return false;
-    // the reference derives all three from non-negative quantities; the
-    // closed-wall ghost (height 1e30) relies on K >= 0
+    // the reference derives all three from non-negative quantities
     if (p->flow_friction < 0.0f || p->water_acceleration < 0.0f ||
         p->cell_area_inv_timescaled < 0.0f)
         return false;
-    return p->boundary == TWS_BOUNDARY_CLOSED || p->boundary == TWS_BOUNDARY_ZERO;
+    // Closed walls: a wall ghost of height 1e30 emits max(0, (1e30 - h) * K),
+    // scaled to 0 by its zero depth; that flux must stay finite (Inf * 0 is
+    // NaN), so K is bounded there.
+    if (p->boundary == TWS_BOUNDARY_CLOSED) return p->water_acceleration <= kMaxClosedK;
+    return p->boundary == TWS_BOUNDARY_ZERO || p->boundary == TWS_BOUNDARY_REFERENCE_DISPATCH;
 }
 
 void fill_ghosts(GridDesc& g, const tws_params& p) {
@@ -60,6 +65,22 @@ void fill_ghosts(GridDesc& g, const tws_params& p) {
 }
 
 int64_t pitch_for(int width) { return ((int64_t)width + 63) / 64 * 64; }
+
+int device_cus() {
+    static std::mutex mu;
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cached[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            return 0;
+        cached[dev] = cus;
+    }
+    return cached[dev];
+}
 
 }  // namespace tws
 
@@ -89,7 +110,8 @@ int tws_params_physical(float dt, float gravity, float cell_size, float pipe_are
     if (!(dt > 0.0f) || !(cell_size > 0.0f) || !std::isfinite(gravity) ||
         !std::isfinite(pipe_area) || !(damping >= 0.0f))
         return fail(TWS_ERR_INVALID, "tws_params_physical: invalid physical parameters");
-    if (boundary != TWS_BOUNDARY_CLOSED && boundary != TWS_BOUNDARY_ZERO)
+    if (boundary != TWS_BOUNDARY_CLOSED && boundary != TWS_BOUNDARY_ZERO &&
+        boundary != TWS_BOUNDARY_REFERENCE_DISPATCH)
         return fail(TWS_ERR_INVALID, "tws_params_physical: unknown boundary %d", boundary);
     tws_params p{};
     p.flow_friction = (float)std::pow((double)damping, (double)dt);
@@ -108,7 +130,8 @@ int tws_params_reference(float steps_per_second, float damping, float accelerati
     if (!out) return fail(TWS_ERR_INVALID, "tws_params_reference: out is NULL");
     if (!(steps_per_second > 0.0f) || grid_resolution == 0 || !(grid_world_size > 0.0f))
         return fail(TWS_ERR_INVALID, "tws_params_reference: invalid settings");
-    if (boundary != TWS_BOUNDARY_CLOSED && boundary != TWS_BOUNDARY_ZERO)
+    if (boundary != TWS_BOUNDARY_CLOSED && boundary != TWS_BOUNDARY_ZERO &&
+        boundary != TWS_BOUNDARY_REFERENCE_DISPATCH)
         return fail(TWS_ERR_INVALID, "tws_params_reference: unknown boundary %d", boundary);
     // Terrain.cpp:177 -- the step length is the float 1/steps, held as double.
     const double dt = (double)(1.0f / steps_per_second);
@@ -129,8 +152,9 @@ int tws_float_to_half_rtz(const float* in, uint16_t* out, int64_t n) {
     return TWS_OK;
 }
 
-int tws_stream_copy_bench(int64_t bytes, int32_t iterations, double* gbps) {
-    if (!gbps || bytes < (1 << 20) || iterations <= 0)
+int tws_stream_copy_bench(int64_t bytes, int32_t iterations, int32_t shape, double* gbps) {
+    if (!gbps || bytes < (1 << 20) || iterations <= 0 || shape < -1 ||
+        shape >= stream_copy_shapes())
         return fail(TWS_ERR_INVALID, "tws_stream_copy_bench: invalid argument");
     bytes &= ~(int64_t)255;
     void *src = nullptr, *dst = nullptr;
@@ -140,22 +164,30 @@ int tws_stream_copy_bench(int64_t bytes, int32_t iterations, double* gbps) {
     if (e == hipSuccess) e = hipMemset(src, 0, (size_t)bytes);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = launch_stream_copy(src, dst, bytes, nullptr);  // warm
-    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
-    for (int i = 0; e == hipSuccess && i < iterations; ++i)
-        e = launch_stream_copy(src, dst, bytes, nullptr);
-    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    const int first = shape < 0 ? 0 : shape, last = shape < 0 ? stream_copy_shapes() - 1 : shape;
+    double best = 0.0;
+    for (int sh = first; e == hipSuccess && sh <= last; ++sh) {
+        e = launch_stream_copy(src, dst, bytes, sh, nullptr);  // warm
+        if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+        for (int i = 0; e == hipSuccess && i < iterations; ++i)
+            e = launch_stream_copy(src, dst, bytes, sh, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms > 0.0f)
+            best = std::max(best, 2.0 * (double)bytes * iterations / (ms * 1e-3) / 1e9);
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (src) (void)hipFree(src);
     if (dst) (void)hipFree(dst);
     if (e != hipSuccess) return hip_fail(e, "tws_stream_copy_bench");
-    *gbps = 2.0 * (double)bytes * iterations / (ms * 1e-3) / 1e9;
+    *gbps = best;
     return TWS_OK;
 }
+
+int tws_stream_copy_shapes(void) { return stream_copy_shapes(); }
 
 int tws_debug_lane_shift_probe(int32_t* out128) {
     if (!out128) return fail(TWS_ERR_INVALID, "tws_debug_lane_shift_probe: NULL");

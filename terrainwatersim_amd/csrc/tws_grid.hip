@@ -48,6 +48,11 @@ GridDesc GridState::desc() const {
     g.status = status;
     g.parity0 = parity;
     g.tile_group = tile_group;
+    const bool dispatch = params.boundary == TWS_BOUNDARY_REFERENCE_DISPATCH;
+    g.upd_w = dispatch ? W / 16 * 16 : W;
+    g.upd_h = dispatch ? H / 16 * 16 : H;
+    g.brush_w = dispatch ? W / 32 * 32 : W;
+    g.brush_h = dispatch ? H / 32 * 32 : H;
     return g;
 }
 

@@ -7,11 +7,14 @@
 
 namespace tws {
 
-// Terrain height given to ghost cells in closed-wall mode.  Any cell next to
-// such a ghost computes a flux toward it of max(0, f*F + (h - kWallHeight)*K),
-// which is +0 for every |f|, |h| below ~1e28, and the ghost itself emits
+// Terrain height given to ghost cells in closed-wall mode.  A cell next to
+// such a ghost computes a flux toward it of max(0, f*F + (h - kWallHeight)*K)
+// = +0 for every K >= 0 once its own stored wall-ward f is 0 (the step
+// kernels zero it when they load an edge cell), and the ghost itself emits
 // (BIG * K) * (0 / S) = 0: closed walls, exactly.
 constexpr float kWallHeight = 1.0e30f;
+// Largest K accepted with closed walls: kWallHeight * K stays finite.
+constexpr float kMaxClosedK = 1.0e8f;
 
 constexpr int kWave = 64;
 
@@ -51,8 +54,21 @@ struct GridDesc {
     float rain_cx, rain_cy, rain_I, rain_R2;
     int32_t rain_on;
     int32_t tile_group;      // tile order: columns per group (0 = row-major)
+    // Cells updated by a step ([0, upd_w) x [0, upd_h)) and reached by the
+    // rain and the brush ([0, brush_w) x [0, brush_h)): the whole grid but in
+    // TWS_BOUNDARY_REFERENCE_DISPATCH mode, where the reference's truncating
+    // dispatch shapes leave the tail untouched.
+    int32_t upd_w, upd_h, brush_w, brush_h;
     unsigned int* status;    // bit 0: non-finite value seen
 };
+
+// Whether a grid must step on the two-pass kernels: the tile kernels update
+// every cell and rain on every cell, so they serve a reference-dispatch grid
+// only when its extents cover the whole grid.
+inline bool needs_twopass(const GridDesc& g) {
+    return g.upd_w < g.W || g.upd_h < g.H ||
+           (g.rain_on != 0 && (g.brush_w < g.W || g.brush_h < g.H));
+}
 
 // Tile geometry of the fused kernel.  CPL columns per lane, RPW rows per
 // wave, NW waves, at most KH steps per launch.
@@ -108,7 +124,26 @@ hipError_t launch_fbm(float* b, const GridDesc& g, uint64_t seed, int octaves, i
                       double persistence, double scale, hipStream_t s);
 hipError_t launch_init_water(const GridDesc& g, int parity, float depth, int x0, int y0, int x1,
                              int y1, float rect_depth, hipStream_t s);
-hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
+// Copy of `bytes` (a multiple of 16) in launch shape `shape`, 0 <= shape <
+// stream_copy_shapes() (loads in flight per lane, non-temporal hints, work
+// groups per CU; shape 0 is the one-load-per-lane grid-stride loop).
+int stream_copy_shapes();
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, int shape,
+                              hipStream_t s);
+
+// The streaming step kernel (tws_stream.hip): one wavefront per strip
+// segment of `seg_rows` output rows and stream_shape() output columns.
+struct StreamPlan {
+    int strips = 0, segs = 0, seg_rows = 0, blocks = 0;
+};
+// Output columns per strip at k steps per launch; false if k is not built.
+bool stream_shape(int k, int* ox);
+// Segments of a W x H grid on a device of `cus` compute units: as many as
+// fill its wavefront slots once.
+StreamPlan stream_plan(int W, int H, int k, int cus);
+// One launch of k steps (g.steps of them) from set `parity`; closed walls only.
+hipError_t launch_stream_single(int k, const GridDesc& g, const StreamPlan& p, int parity,
+                                hipStream_t s);
 
 constexpr int kVolumeParts = 512;
 

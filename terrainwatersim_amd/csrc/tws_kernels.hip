@@ -20,124 +20,14 @@
 // compiled with -ffp-contract=off, so each cell's result is bit-identical to
 // the FP32 restatement regardless of tile shape or steps per launch.
 
+#include <algorithm>
+
 #include "tws_internal.hpp"
 #include "tws.h"
 #include "tws_fbm.hpp"
+#include "tws_kernel_util.hpp"
 
 namespace tws {
-
-// ---------------------------------------------------------------------------
-// Cross-lane moves.  DPP wave_shr:1 (lane i reads lane i-1) and wave_shl:1
-// (lane i reads lane i+1).  The lane without a source (0 resp. 63, which sit
-// in the tile halo) reads 0, or `fill` in the *_or variants: heads are filled
-// with the lane's own head so the halo edge sees no gradient and does not
-// send every wave down the flux-scaling division.
-__device__ __forceinline__ float from_left_lane(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float from_right_lane(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float from_left_lane_or(float v, float fill) {
-    return __int_as_float(
-        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float from_right_lane_or(float v, float fill) {
-    return __int_as_float(
-        __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xF, 0xF, false));
-}
-
-// Global-address-space views of the state planes: loads and stores through
-// them are global_* instructions rather than flat_* ones.
-typedef __attribute__((address_space(1))) float gfloat;
-__device__ __forceinline__ const gfloat* gptr(const float* p) { return (const gfloat*)p; }
-__device__ __forceinline__ gfloat* gptr(float* p) { return (gfloat*)p; }
-
-typedef float vf2 __attribute__((ext_vector_type(2)));
-typedef float vf4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) vf2 gfloat2;
-typedef __attribute__((address_space(1))) vf4 gfloat4;
-
-template <int N>
-__device__ __forceinline__ void vload(const gfloat* p, float (&o)[N]);
-template <>
-__device__ __forceinline__ void vload<1>(const gfloat* p, float (&o)[1]) { o[0] = p[0]; }
-template <>
-__device__ __forceinline__ void vload<2>(const gfloat* p, float (&o)[2]) {
-    const vf2 v = *reinterpret_cast<const gfloat2*>(p);
-    o[0] = v.x; o[1] = v.y;
-}
-template <>
-__device__ __forceinline__ void vload<4>(const gfloat* p, float (&o)[4]) {
-    const vf4 v = *reinterpret_cast<const gfloat4*>(p);
-    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-}
-// Stores of the new state are marked non-temporal: no tile of the same
-// launch reads them back, so they need not displace the halos other tiles
-// are about to re-read from L2 (measured +1.5-3% at 4096^2).
-template <int N>
-__device__ __forceinline__ void vstore(gfloat* p, const float (&o)[N]);
-template <>
-__device__ __forceinline__ void vstore<1>(gfloat* p, const float (&o)[1]) { p[0] = o[0]; }
-template <>
-__device__ __forceinline__ void vstore<2>(gfloat* p, const float (&o)[2]) {
-    __builtin_nontemporal_store(vf2{o[0], o[1]}, reinterpret_cast<gfloat2*>(p));
-}
-template <>
-__device__ __forceinline__ void vstore<4>(gfloat* p, const float (&o)[4]) {
-    __builtin_nontemporal_store(vf4{o[0], o[1], o[2], o[3]}, reinterpret_cast<gfloat4*>(p));
-}
-
-// Non-short-circuit finiteness test (no branches).
-__device__ __forceinline__ bool nonfinite5(float a, float b, float c, float d, float e) {
-    return !__builtin_isfinite(a) | !__builtin_isfinite(b) | !__builtin_isfinite(c) |
-           !__builtin_isfinite(d) | !__builtin_isfinite(e);
-}
-
-// waterBrush.comp:26-28, one texel: saturate(1 - dot(c-p, c-p)/R^2) * I.
-__device__ __forceinline__ float brush_amount(float cx, float cy, float I, float R2, int x, int y) {
-    const float tx = cx - (float)x;
-    const float ty = cy - (float)y;
-    const float dist = tx * tx + ty * ty;
-    const float q = dist / R2;
-    float w = 1.0f - q;
-    w = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
-    return w * I;
-}
-
-// Whether the brush can reach any cell of columns [x_lo, x_hi] x rows
-// [y_lo, y_hi].  brush_amount is +0 * I, which leaves every depth unchanged
-// but for the sign of a zero, wherever the FP32 squared distance is at least
-// R^2.  That distance is within 4 roundings (a relative 2.4e-7) of the exact
-// one, so a block whose exact nearest squared distance is at least
-// R^2 * (1 + 1e-6) gets +0 * I at every cell.
-__device__ __forceinline__ bool brush_reaches(float cx, float cy, float R2, int x_lo, int x_hi,
-                                              int y_lo, int y_hi) {
-    const double dx = cx < (float)x_lo ? (double)x_lo - cx : (cx > (float)x_hi ? (double)cx - x_hi : 0.0);
-    const double dy = cy < (float)y_lo ? (double)y_lo - cy : (cy > (float)y_hi ? (double)cy - y_hi : 0.0);
-    return dx * dx + dy * dy < (double)R2 * (1.0 + 1e-6);
-}
-
-// flowUpdate.comp:53-59 for one cell: new outgoing flux from the four head
-// differences, clamped at zero and scaled so the cell cannot drain below 0.
-__device__ __forceinline__ void flux_update(float F, float K, float C, float d, float g0,
-                                            float g1, float g2, float g3, float& f0, float& f1,
-                                            float& f2, float& f3) {
-    float o0 = fmaxf(0.0f, f0 * F + g0 * K);
-    float o1 = fmaxf(0.0f, f1 * F + g1 * K);
-    float o2 = fmaxf(0.0f, f2 * F + g2 * K);
-    float o3 = fmaxf(0.0f, f3 * F + g3 * K);
-    // flowUpdate.comp:57 summed pairwise, the order Mesa executes the reference in
-    const float total = ((o0 + o1) + (o2 + o3)) * C;
-    if (total > d) {
-        // d == 0 (a dry cell on a slope) gives s = 0/total = +0 exactly, so the
-        // division is only executed where some lane holds water.
-        float s = 0.0f;
-        if (d != 0.0f) s = d / total;
-        o0 = o0 * s; o1 = o1 * s; o2 = o2 * s; o3 = o3 * s;
-    }
-    f0 = o0; f1 = o1; f2 = o2; f3 = o3;
-}
 
 // ---------------------------------------------------------------------------
 // Tile (tx, ty) of a grid's tile list position lt: groups of tile_group tile
@@ -237,8 +127,27 @@ __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int
             }
         }
     }
-    const bool need_mask = EDGE && g.boundary == TWS_BOUNDARY_ZERO &&
-                           (ox < 0 || oy < 0 || ox + EXW > W || oy + EXH > H);
+    const bool edge_tile = ox < 0 || oy < 0 || ox + EXW > W || oy + EXH > H;
+    const bool need_mask = EDGE && g.boundary != TWS_BOUNDARY_CLOSED && edge_tile;
+    // Closed walls carry no flux either way.  The wall ghost makes every
+    // wall-ward flux the kernel computes max(0, f*F + (h - 1e30)*K) = +0 as
+    // long as the cell's own stored wall-ward f is 0; an uploaded state may
+    // hold another value there (and with K = 0 it would pass the wall), so
+    // edge cells drop it as they load.
+    if ((!EDGE || g.boundary == TWS_BOUNDARY_CLOSED) && edge_tile) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int y = oy + ly + r;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const int x = x0 + c;
+                if (x == W - 1) f0[r][c] = 0.0f;
+                if (x == 0) f1[r][c] = 0.0f;
+                if (y == H - 1) f2[r][c] = 0.0f;
+                if (y == 0) f3[r][c] = 0.0f;
+            }
+        }
+    }
     // The brush amounts do not change from level to level: computed once, in
     // registers across the loop (recomputing them per level measured 13%
     // slower at 256^2, where waves under the disc are on the critical path),
@@ -461,6 +370,10 @@ __global__ void __launch_bounds__(256) k_flux_twopass(GridDesc g, int parity) {
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= g.W || y >= g.H) return;
     const int64_t p = g.pitch, i = (int64_t)y * p + x;
+    if (x >= g.upd_w || y >= g.upd_h) {  // no reference work group covers it
+        for (int c = 0; c < 4; ++c) g.f[parity ^ 1][c][i] = g.f[parity][c][i];
+        return;
+    }
     const float* b = g.b;
     const float* d = g.d[parity];
     const bool in_x1 = x + 1 < g.W, in_x0 = x > 0, in_y1 = y + 1 < g.H, in_y0 = y > 0;
@@ -503,6 +416,10 @@ __global__ void __launch_bounds__(256) k_apply_twopass(GridDesc g, int parity) {
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= g.W || y >= g.H) return;
     const int64_t p = g.pitch, i = (int64_t)y * p + x;
+    if (x >= g.upd_w || y >= g.upd_h) {  // no reference work group covers it
+        g.d[parity ^ 1][i] = g.d[parity][i];
+        return;
+    }
     float* const* fo = g.f[parity ^ 1];
     const float gz = g.boundary == TWS_BOUNDARY_CLOSED ? 0.0f : g.ghost_fw;
     const float X1y = x + 1 < g.W ? fo[1][i + 1] : 0.0f;
@@ -519,7 +436,7 @@ __global__ void __launch_bounds__(256) k_brush(GridDesc g, int parity, float cx,
                                                 float I, float R2) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= g.W || y >= g.H) return;
+    if (x >= g.brush_w || y >= g.brush_h) return;
     const int64_t i = (int64_t)y * g.pitch + x;
     float* d = g.d[parity];
     d[i] = d[i] + brush_amount(cx, cy, I, R2, x, y);
@@ -539,8 +456,10 @@ __global__ void __launch_bounds__(256) k_velocity(GridDesc g, int parity, float*
     const float Y1w = y + 1 < g.H ? f[3][i + p] : gz;
     const float Y0z = y > 0 ? f[2][i - p] : 0.0f;
     const int64_t o = (int64_t)y * g.W + x;
-    v2[o * 2 + 0] = (X1y - f[0][i]) - (X0x - f[1][i]);
-    v2[o * 2 + 1] = (Y1w - f[2][i]) - (Y0z - f[3][i]);
+    // a cell no reference work group covers never receives a FlowMap value
+    const bool upd = x < g.upd_w && y < g.upd_h;
+    v2[o * 2 + 0] = upd ? (X1y - f[0][i]) - (X0x - f[1][i]) : 0.0f;
+    v2[o * 2 + 1] = upd ? (Y1w - f[2][i]) - (Y0z - f[3][i]) : 0.0f;
 }
 
 // Planar flux of set `parity` <-> dense interleaved staging[(x + y*W)*4 + c].
@@ -599,8 +518,9 @@ __global__ void __launch_bounds__(256) k_ref_pack(GridDesc g, int parity, float 
         const float X0x = x > 0 ? f[0][i - 1] : 0.0f;
         const float Y1w = y + 1 < g.H ? f[3][i + p] : gz;
         const float Y0z = y > 0 ? f[2][i - p] : 0.0f;
-        flowmap[o * 2 + 0] = half_rtz((X1y - f[0][i]) - (X0x - f[1][i]));
-        flowmap[o * 2 + 1] = half_rtz((Y1w - f[2][i]) - (Y0z - f[3][i]));
+        const bool upd = x < g.upd_w && y < g.upd_h;  // see k_velocity
+        flowmap[o * 2 + 0] = upd ? half_rtz((X1y - f[0][i]) - (X0x - f[1][i])) : (uint16_t)0;
+        flowmap[o * 2 + 1] = upd ? half_rtz((Y1w - f[2][i]) - (Y0z - f[3][i])) : (uint16_t)0;
     }
 }
 
@@ -668,13 +588,35 @@ __global__ void __launch_bounds__(256) k_init_water(GridDesc g, int parity, floa
     for (int c = 0; c < 4; ++c) g.f[parity][c][i] = 0.0f;
 }
 
-// Plain device copy of n float4 (STREAM copy): the measured HBM peak, and
-// the known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters.
-__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src,
-                                                      float4* __restrict__ dst, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        dst[i] = src[i];
+// Device copy of n float4 (STREAM copy): the measured HBM ceiling, and the
+// known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters.
+// Each lane keeps U loads in flight (issued back to back, then U stores);
+// NT marks them non-temporal.  Chunks of U * blockDim float4 are dealt out
+// grid-stride; the tail chunk is bounds-checked per element.
+template <int U, bool NT>
+__global__ void __launch_bounds__(256) k_stream_copy(const vf4* __restrict__ src,
+                                                      vf4* __restrict__ dst, int64_t n) {
+    const int64_t chunk = (int64_t)U * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
+        vf4 v[U];
+        if (base + chunk <= n) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+                v[u] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+                if (NT) __builtin_nontemporal_store(v[u], dst + i); else dst[i] = v[u];
+            }
+        } else {
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+                if (i < n) dst[i] = src[i];
+            }
+        }
+    }
 }
 
 // Lane-layout check of the DPP wave shifts used by k_tile.
@@ -781,7 +723,7 @@ hipError_t launch_tile_single(int kernel, int k, const GridDesc& g, int launch_i
 
 hipError_t launch_twopass(const GridDesc* /*g_dev*/, const GridDesc& g, int parity,
                           hipStream_t s) {
-    if (g.rain_on)
+    if (g.rain_on && g.brush_w > 0 && g.brush_h > 0)
         hipLaunchKernelGGL(k_brush, cell_grid(g), dim3(256), 0, s, g, parity, g.rain_cx, g.rain_cy,
                            g.rain_I, g.rain_R2);
     hipLaunchKernelGGL(k_flux_twopass, cell_grid(g), dim3(256), 0, s, g, parity);
@@ -858,13 +800,35 @@ hipError_t launch_check_finite(const GridDesc& g, int parity, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
+// Launch shapes of the copy: (loads in flight per lane, non-temporal,
+// work groups of 256 per CU).
+struct CopyShape {
+    int unroll;
+    bool nt;
+    int groups_per_cu;
+};
+static const CopyShape kCopyShapes[] = {
+    {1, false, 64},  // one load per lane and iteration
+    {4, false, 8}, {4, true, 8}, {8, true, 4}, {8, true, 8}, {4, true, 16}, {8, false, 8},
+};
+int stream_copy_shapes() { return (int)(sizeof(kCopyShapes) / sizeof(kCopyShapes[0])); }
+
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, int shape,
+                              hipStream_t s) {
+    if (shape < 0 || shape >= stream_copy_shapes()) return hipErrorInvalidValue;
+    const CopyShape& c = kCopyShapes[shape];
     const int64_t n = bytes / 16;
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 256 * 64) blocks = 256 * 64;
-    if (blocks < 1) return hipSuccess;
-    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s,
-                       static_cast<const float4*>(src), static_cast<float4*>(dst), n);
+    if (n < 1) return hipSuccess;
+    int64_t blocks = (n + 256LL * c.unroll - 1) / (256LL * c.unroll);
+    blocks = std::min<int64_t>(blocks, 256LL * c.groups_per_cu);
+    const vf4* sv = static_cast<const vf4*>(src);
+    vf4* dv = static_cast<vf4*>(dst);
+    const dim3 grid((unsigned)blocks), block(256);
+    if (c.unroll == 1) hipLaunchKernelGGL((k_stream_copy<1, false>), grid, block, 0, s, sv, dv, n);
+    else if (c.unroll == 4 && !c.nt) hipLaunchKernelGGL((k_stream_copy<4, false>), grid, block, 0, s, sv, dv, n);
+    else if (c.unroll == 4) hipLaunchKernelGGL((k_stream_copy<4, true>), grid, block, 0, s, sv, dv, n);
+    else if (!c.nt) hipLaunchKernelGGL((k_stream_copy<8, false>), grid, block, 0, s, sv, dv, n);
+    else hipLaunchKernelGGL((k_stream_copy<8, true>), grid, block, 0, s, sv, dv, n);
     return hipGetLastError();
 }
 

@@ -105,7 +105,7 @@ constexpr int kTailThreads = 1024;
 constexpr int kTailTexels = 64 * 64;
 
 __global__ void __launch_bounds__(kTailThreads)
-k_mip_tail(const float4* __restrict__ src, int ws, int hs, float4* __restrict__ dst) {
+k_mip_tail(const float4* src, int ws, int hs, float4* dst) {
     while (ws > 1 || hs > 1) {
         const int wd = ws / 2 > 0 ? ws / 2 : 1, hd = hs / 2 > 0 ? hs / 2 : 1;
         for (int k = threadIdx.x; k < wd * hd; k += kTailThreads) {

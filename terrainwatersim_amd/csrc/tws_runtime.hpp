@@ -16,6 +16,15 @@ int hip_fail(hipError_t e, const char* what);
 int64_t pitch_for(int width);
 void fill_ghosts(GridDesc& g, const tws_params& p);
 bool valid_params(const tws_params* p);
+// Whether a batch can step a W x H grid with these parameters: a
+// reference-dispatch grid whose dispatch shapes leave a tail (W or H not a
+// multiple of 32) steps on the single-grid path only.
+inline bool batchable(int W, int H, const tws_params& p) {
+    return p.boundary != TWS_BOUNDARY_REFERENCE_DISPATCH || (W % 32 == 0 && H % 32 == 0);
+}
+
+// Compute units of the current device, or 0 if it cannot be queried.
+int device_cus();
 
 // Launch configuration (kernel variant, steps per launch) for a grid size,
 // from the tuning table in tws_tune.cpp.

@@ -401,6 +401,9 @@ int tws_server_submit(tws_server* s, const tws_request* r, uint64_t* ticket) {
     if (!s || !r || !ticket) return fail(TWS_ERR_INVALID, "tws_server_submit: NULL argument");
     if (r->width <= 0 || r->height <= 0 || r->steps < 0 || !r->terrain || !valid_params(&r->params))
         return fail(TWS_ERR_INVALID, "tws_server_submit: invalid request");
+    if (!batchable(r->width, r->height, r->params))
+        return fail(TWS_ERR_INVALID, "tws_server_submit: a reference-dispatch grid of %d x %d (not "
+                    "multiples of 32) steps on the single-grid path only", r->width, r->height);
     if (r->rain_on && (!(r->rain[3] > 0.0f) || !std::isfinite(r->rain[0]) ||
                        !std::isfinite(r->rain[1]) || !std::isfinite(r->rain[2])))
         return fail(TWS_ERR_INVALID, "tws_server_submit: invalid rain");

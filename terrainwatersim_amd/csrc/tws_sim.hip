@@ -35,6 +35,37 @@ struct tws_sim {
 
 namespace tws {
 
+// Streaming launches (tws_stream.hip): full launches of k steps, then the
+// remainder as launches of the largest built step counts that fit it (each
+// launch runs exactly its template's number of levels).
+static int step_stream(GridState& gs, int k, int32_t n, hipStream_t stream, hipEvent_t* ev,
+                       int ev_pairs, int* launches) {
+    int nl = 0;
+    const int cus = device_cus();
+    if (cus <= 0) return fail(TWS_ERR_DEVICE, "step: cannot query the device");
+    int left = n;
+    while (left > 0) {
+        int kk = k;
+        if (left < kk) {
+            static const int ladder[] = {8, 6, 5, 4, 3, 2, 1};
+            for (int c : ladder)
+                if (c <= left) { kk = c; break; }
+        }
+        const StreamPlan plan = stream_plan(gs.W, gs.H, kk, cus);
+        if (plan.blocks <= 0) return fail(TWS_ERR_INVALID, "step: bad stream launch (k %d)", kk);
+        GridDesc g = gs.desc();
+        g.steps = kk;
+        if (ev && nl < ev_pairs) TWS_HIP(hipEventRecord(ev[2 * nl], stream), "step: event");
+        TWS_HIP(launch_stream_single(kk, g, plan, gs.parity, stream), "step: stream kernel");
+        if (ev && nl < ev_pairs) TWS_HIP(hipEventRecord(ev[2 * nl + 1], stream), "step: event");
+        gs.parity ^= 1;
+        left -= kk;
+        ++nl;
+    }
+    if (launches) *launches = nl;
+    return TWS_OK;
+}
+
 // Runs n steps of one grid as a sequence of launches: full launches of k
 // steps, then the tail on the smallest template that holds it.  With `ev`
 // (2 events per launch) each launch is bracketed by an event pair.
@@ -46,7 +77,13 @@ int step_grid(GridState& gs, int kernel, int k, int32_t n, hipStream_t stream, h
         if (!ev || nl >= ev_pairs) return hipSuccess;
         return hipEventRecord(ev[2 * nl + which], stream);
     };
-    if (kernel == TWS_KERNEL_TWOPASS) {
+    if (kernel == TWS_KERNEL_STREAM_128) {
+        // closed walls only; other edges take the large-grid tile kernel
+        if (gs.params.boundary == TWS_BOUNDARY_CLOSED)
+            return step_stream(gs, k, n, stream, ev, ev_pairs, launches);
+        kernel = TWS_KERNEL_TILE_128x64;
+    }
+    if (kernel == TWS_KERNEL_TWOPASS || needs_twopass(gs.desc())) {
         for (int i = 0; i < n; ++i) {
             const GridDesc g = gs.desc();
             TWS_HIP(rec(0), "step: event");
@@ -321,6 +358,10 @@ int tws_set_launch(tws_sim* s, int32_t kernel, int32_t k) {
     }
     if (kernel == TWS_KERNEL_TWOPASS) {
         if (k != 1) return fail(TWS_ERR_INVALID, "tws_set_launch: TWOPASS runs 1 step per launch");
+    } else if (kernel == TWS_KERNEL_STREAM_128) {
+        int ox = 0;
+        if (k < 2 || !stream_shape(k, &ox))
+            return fail(TWS_ERR_INVALID, "tws_set_launch: STREAM_128 runs k = 2..6 or 8, not %d", k);
     } else {
         TileShape sh{};
         if (!tile_shape(kernel, k, &sh))

@@ -6,10 +6,12 @@
 // 64 x 64 tiles (16 waves up to 512^2, 8 waves up to 2048^2: small grids
 // are bound by the latency of one tile's levels, so more and smaller tiles
 // finish sooner), 128 x 64 tiles beyond (two work groups per CU so one loads
-// while the other computes), 5 steps per launch up to 4096^2 and 4 above.  tws_autotune()
-// times every candidate on scratch grids of the requested shapes and records
-// the winner for that size class (log2 of the cell count), which
-// choose_launch() then prefers.
+// while the other computes), 5 steps per launch up to 4096^2 and 4 above.
+// tws_autotune() times every candidate on scratch grids of the requested
+// shapes and returns the winner; only tws_tune_record() enters a launch into
+// the per-size-class table (log2 of the cell count) that choose_launch()
+// consults first, so tuning a batch never changes what a later plain grid
+// of the process runs.
 
 #include <hip/hip_runtime.h>
 
@@ -207,14 +209,20 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
     if (rc != TWS_OK) return rc;
     if (count) *count = m;
     *best = top;
-    {
-        // record for the size class of the typical grid of the set
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        try {
-            g_tuned[size_class(cells / n)] = {top.kernel, top.steps_per_launch};
-        } catch (...) {
-            return fail(TWS_ERR_NOMEM, "tws_autotune: recording the result");
-        }
+    return TWS_OK;
+}
+
+int tws_tune_record(int32_t width, int32_t height, int32_t kernel, int32_t k) {
+    if (width <= 0 || height <= 0)
+        return fail(TWS_ERR_INVALID, "tws_tune_record: invalid size %d x %d", width, height);
+    TileShape sh{};
+    if (!tile_shape(kernel, k, &sh))
+        return fail(TWS_ERR_INVALID, "tws_tune_record: unknown kernel %d or k %d", kernel, k);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    try {
+        g_tuned[size_class((int64_t)width * height)] = {kernel, k};
+    } catch (...) {
+        return fail(TWS_ERR_NOMEM, "tws_tune_record: host allocation failed");
     }
     return TWS_OK;
 }

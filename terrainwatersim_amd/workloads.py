@@ -41,27 +41,36 @@ def _splitmix(x: int) -> int:
 
 
 # ---------------------------------------------------------------------------
-def small(steps=1000, repeats=5, log=print):
-    p = tws.params_physical(**PARAMS)
+def small_state(steps=1000):
+    """The small workload's initial state: (b, d, rain, steps)."""
     N = 256
     b = tws.generate_fbm_terrain(N, N, seed=1)
+    return b, np.zeros((N, N), F32), (128.0, 128.0, 1e-3, 256.0), steps
+
+
+def small(steps=1000, repeats=5, log=print, return_depth=False):
+    p = tws.params_physical(**PARAMS)
+    b, d0, rain, steps = small_state(steps)
+    N = b.shape[0]
     log("small: 256^2 rain, 1000 steps")
     times = []
     with tws.Simulation(N, N, p) as sim:
         kernel, k = sim.get_launch()
         for _ in range(repeats + 1):
-            sim.upload(b, np.zeros((N, N), F32))
-            sim.set_rain(128.0, 128.0, 1e-3, 256.0)
+            sim.upload(b, d0)
+            sim.set_rain(*rain)
             sim.sync()
             t0 = time.perf_counter()
             sim.step(steps)
             sim.sync()
             times.append(time.perf_counter() - t0)
         vol = sim.volume()
+        depth = sim.download(flux=False)["depth"]
     t = float(np.median(times[1:]))
-    return {"grid": N, "steps": steps, "kernel": kernel, "steps_per_launch": k,
-            "cell_updates_per_s": N * N * steps / t, "ms_per_1000_steps": t * 1e3 * 1000 / steps,
-            "volume_m3": vol}
+    rec = {"grid": N, "steps": steps, "kernel": kernel, "steps_per_launch": k,
+           "cell_updates_per_s": N * N * steps / t, "ms_per_1000_steps": t * 1e3 * 1000 / steps,
+           "volume_m3": vol}
+    return (rec, depth) if return_depth else rec
 
 
 def _draw_sides(rng, n, lo, hi):
@@ -123,13 +132,40 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
         out["batched_ms_per_run"] = t * 1e3
         out["graph"] = batch.stats()
         checks = [batch.download(g, flux=False)["depth"] for g in ids[:4]]
-    # the same grids, one at a time (each stepped and synchronised alone)
-    log("batched: one grid at a time")
-    sims = []
+    # The same grids one at a time, each stepped and synchronised alone: first
+    # on its own default launch (what a caller of the plain path gets; the
+    # tuner's table is cleared so the batch's result does not leak into it),
+    # then forced onto the batch's tuned launch.
+    for leg, forced in (("plain", False), ("plain_forced", True)):
+        log(f"batched: one grid at a time ({'batch launch' if forced else 'default launches'})")
+        tws.tune_forget()
+        t, same, launches = _one_at_a_time(grids, p, steps, repeats, checks,
+                                           (best["kernel"], best["k"]) if forced else None)
+        out[f"{leg}_cell_updates_per_s"] = cells * steps / t
+        out[f"{leg}_ms_per_run"] = t * 1e3
+        out[f"{leg}_equals_batched"] = bool(same)
+        if not forced:
+            out["plain_launches"] = launches
+    out["batched_over_plain"] = out["batched_cell_updates_per_s"] / out["plain_cell_updates_per_s"]
+    out["batched_over_plain_forced"] = (out["batched_cell_updates_per_s"]
+                                        / out["plain_forced_cell_updates_per_s"])
+    out["batched_equals_plain"] = bool(out["plain_equals_batched"]
+                                       and out["plain_forced_equals_batched"])
+    return out
+
+
+def _one_at_a_time(grids, p, steps, repeats, checks, launch=None):
+    """Median seconds to step every grid alone (one Simulation each, stepped
+    and synchronised in turn), whether the first grids' depths equal
+    `checks`, and the launches used ({"kernel/k": grids})."""
+    sims, launches = [], {}
     try:
         for s, b, rain in grids:
             sim = tws.Simulation(s, s, p)
-            sim.set_launch(best["kernel"], best["k"])
+            if launch is not None:
+                sim.set_launch(*launch)
+            kl = "%d/%d" % sim.get_launch()
+            launches[kl] = launches.get(kl, 0) + 1
             sim.upload(b, np.zeros((s, s), F32))
             sim.set_rain(*rain)
             sims.append(sim)
@@ -143,17 +179,12 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
                 sim.step(steps)
                 sim.sync()
             times.append(time.perf_counter() - t0)
-        t = float(np.median(times))
-        out["plain_cell_updates_per_s"] = cells * steps / t
-        out["plain_ms_per_run"] = t * 1e3
         same = all(np.array_equal(c, sim.download(flux=False)["depth"])
-                   for c, sim in zip(checks, sims[:4]))
+                   for c, sim in zip(checks, sims[:len(checks)]))
     finally:
         for sim in sims:
             sim.close()
-    out["batched_over_plain"] = out["batched_cell_updates_per_s"] / out["plain_cell_updates_per_s"]
-    out["batched_equals_plain"] = bool(same)
-    return out
+    return float(np.median(times)), same, launches
 
 
 # ---------------------------------------------------------------------------

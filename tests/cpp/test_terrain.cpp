@@ -1,8 +1,14 @@
 // Drives the C++ mirror of the reference's Terrain class (include/tws_terrain.hpp)
 // and checks it against the same steps issued through the C ABI directly.
 // Built by __graft_entry__.build(); run by tests/test_terrain_cpp.py.
+//   test_terrain [N [out_prefix]]
+// N: grid resolution (default 256).  With out_prefix, the initial terrain and
+// depth and the final depth and flux are written as raw float32 files
+// <out_prefix>{b0,d0,d,f}.f32 for a comparison with the CPU restatement.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "tws_terrain.hpp"
@@ -13,8 +19,17 @@ static int failures = 0;
         if (!(c)) { std::printf("FAILED: %s (line %d)\n", #c, __LINE__); ++failures; } \
     } while (0)
 
-int main() {
-    const int N = 256;
+static bool write_f32(const std::string& path, const std::vector<float>& v) {
+    FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    const bool ok = std::fwrite(v.data(), sizeof(float), v.size(), fp) == v.size();
+    return std::fclose(fp) == 0 && ok;
+}
+
+int main(int argc, char** argv) {
+    const int N = argc > 1 ? std::atoi(argv[1]) : 256;
+    const char* out_prefix = argc > 2 ? argv[2] : nullptr;
+    if (N < 16) { std::printf("N must be at least 16\n"); return 2; }
     tws::Terrain terrain(N, 1024.0f);
     EXPECT(terrain.Ok());
     // Terrain.h:76 returns (float)(1/step + 0.5) with step = (double)(1.0f/60):
@@ -62,7 +77,10 @@ int main() {
 
     // the same through the C ABI
     tws_params p{};
-    EXPECT(tws_params_reference(60.0f, 0.98f, 10.0f, 1024.0f, N, TWS_BOUNDARY_CLOSED, &p) == TWS_OK);
+    // the mirror's default edge: the shaders' own, with their dispatch shapes
+    EXPECT(terrain.Params().boundary == TWS_BOUNDARY_REFERENCE_DISPATCH);
+    EXPECT(tws_params_reference(60.0f, 0.98f, 10.0f, 1024.0f, N, TWS_BOUNDARY_REFERENCE_DISPATCH,
+                                &p) == TWS_OK);
     tws_sim* s = nullptr;
     EXPECT(tws_create(N, N, &p, nullptr, &s) == TWS_OK);
     EXPECT(tws_upload(s, b.data(), d.data(), nullptr) == TWS_OK);
@@ -71,7 +89,8 @@ int main() {
     EXPECT(tws_apply_brush(s, u * N, (v - 0.0f) * N, 2.0f, 32.0f) == TWS_OK);
     EXPECT(tws_step(s, 10) == TWS_OK);
     tws_params p2{};
-    EXPECT(tws_params_reference(60.0f, 0.9f, 10.0f, 1024.0f, N, TWS_BOUNDARY_CLOSED, &p2) == TWS_OK);
+    EXPECT(tws_params_reference(60.0f, 0.9f, 10.0f, 1024.0f, N, TWS_BOUNDARY_REFERENCE_DISPATCH,
+                                &p2) == TWS_OK);
     EXPECT(tws_set_params(s, &p2) == TWS_OK);
     EXPECT(tws_step(s, 1) == TWS_OK);
     std::vector<float> dd(N * N), ff(N * N * 4);
@@ -87,14 +106,23 @@ int main() {
             if (f1[i * 4 + c] != ff[i * 4 + c]) ++diff;
     }
     EXPECT(diff == 0);
+    if (out_prefix) {
+        std::vector<float> f_planar(ff);
+        const std::string pre(out_prefix);
+        EXPECT(write_f32(pre + "b0.f32", b) && write_f32(pre + "d0.f32", d) &&
+               write_f32(pre + "d.f32", dd) && write_f32(pre + "f.f32", f_planar));
+    }
 
-    // mip chain of the current TerrainData: 9 levels for 256^2 (Texture.cpp:28-44),
+    // mip chain of the current TerrainData: floor(log2 N) + 1 levels (Texture.cpp:28-44),
     // level 1 = 128^2 texels of (r, 0.3, 0.3, a), each the half-weight bilinear
     // blend of its 2x2 block of level 0
     std::vector<float> mips;
     terrain.ReadMipChain(&mips);
     size_t want = 0;
-    for (int l = 1; l < 9; ++l) want += 4u * (N >> l) * (N >> l);
+    int levels = 1;
+    for (int m = N; m > 1; m >>= 1) ++levels;
+    for (int l = 1; l < levels; ++l)
+        want += 4u * (size_t)std::max(1, N >> l) * (size_t)std::max(1, N >> l);
     EXPECT(terrain.MipChainFloats() == want && mips.size() == want);
     int mip_bad = 0;
     auto lerp = [](float a, float b2) { return a + 0.5f * (b2 - a); };

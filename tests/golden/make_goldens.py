@@ -13,6 +13,16 @@ frame, max(0, f*F + (h - 1e6)*K), is exactly 0, and frame cells (d = 0)
 scale their own outflow by 0/S = 0.  The reference-edge mode runs the domain
 unpadded, so the shaders read texels outside the image.
 
+The reference-dispatch case runs the unmodified dispatch shapes on a grid
+whose sides are not multiples of 16 or 32 (Terrain.cpp:167,258,264 divide
+the resolution with truncation), so the fixture records which cells the
+reference leaves untouched.
+
+The window case (window_cases in the manifest) pins the headline
+configuration directly: the 4096^2 fBm terrain (seed 2) under a 0.5 m sheet,
+closed walls, 64 steps; only the centre 256^2 window of the result is
+stored, with the SHA-256 of the full terrain it started from.
+
 Also records the llvmpipe timing of the 256^2 rain workload (1000 steps) and
 of 20 steps of a 4096^2 grid as profiles/cpu_baseline_llvmpipe.json.
 
@@ -23,6 +33,7 @@ glGenerateMipmap on a full-chain RGBA32F texture (r = b, g = b = 0.3,
 a = d), levels 1.. read back from llvmpipe.
 
     python tests/golden/make_goldens.py [--shaders DIR] [--skip-timing] [--mips-only]
+                                        [--cases NAME,...] [--window]
 """
 from __future__ import annotations
 
@@ -173,6 +184,38 @@ def write_mips(gl, tws):
     return entries
 
 
+WINDOW = 256
+
+
+def write_window(gl, tws, FKC, N=4096, steps=64):
+    """The headline configuration (4096^2 fBm seed 2 + 0.5 m sheet, closed
+    walls) after `steps` steps of the shaders; the centre WINDOW^2 of d, f
+    and the FlowMap bits is stored."""
+    b = tws.generate_fbm_terrain(N, N, seed=2, octaves=6, base_frequency=4, persistence=0.43,
+                                 height_scale=100.0)
+    d = np.full((N, N), 0.5, F32)
+    t0 = time.time()
+    out = run_case(gl, b, d, None, FKC, (steps,), "closed")
+    y0 = x0 = (N - WINDOW) // 2
+    win = lambda a: np.ascontiguousarray(a[y0:y0 + WINDOW, x0:x0 + WINDOW])
+    arrays = {"b_window": win(b), f"d_{steps}": win(out[f"d_{steps}"]),
+              f"f_{steps}": win(out[f"f_{steps}"]), f"v16_{steps}": win(out[f"v16_{steps}"])}
+    name = f"config2_{N}_window{steps}"
+    np.savez_compressed(HERE / f"{name}.npz", **arrays)
+    print(f"{name}: {time.time() - t0:.1f}s", flush=True)
+    return {name: {
+        "shape": [N, N], "boundary": "closed", "steps": [steps],
+        "initial": "terrain generate_fbm_terrain(4096, 4096, seed=2, octaves=6, base_frequency=4, "
+                   "persistence=0.43, height_scale=100); depth 0.5 m everywhere; zero flux",
+        "terrain_sha256": hashlib.sha256(b.tobytes()).hexdigest(),
+        "window": [y0, x0, WINDOW, WINDOW],
+        "F": bits(FKC[0]), "K": bits(FKC[1]), "C": bits(FKC[2]),
+        "F_value": float(FKC[0]), "K_value": float(FKC[1]), "C_value": float(FKC[2]),
+        "arrays": {k: hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest()
+                   for k, v in arrays.items()},
+    }}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--shaders", default=os.environ.get(
@@ -180,6 +223,10 @@ def main():
     ap.add_argument("--skip-timing", action="store_true")
     ap.add_argument("--mips-only", action="store_true",
                     help="only (re)write the mip fixtures and their manifest entries")
+    ap.add_argument("--cases", default="",
+                    help="comma list: only (re)write these step cases (others kept as they are)")
+    ap.add_argument("--window", action="store_true",
+                    help="only (re)write the 4096^2 centre-window case")
     args = ap.parse_args()
 
     import terrainwatersim_amd as tws   # host-only functions: params, terrain generator
@@ -202,6 +249,11 @@ def main():
     if args.mips_only:
         old = json.loads((HERE / "manifest.json").read_text())
         old["mip_cases"] = write_mips(gl, tws)
+        (HERE / "manifest.json").write_text(json.dumps(old, indent=1) + "\n")
+        return
+    if args.window:
+        old = json.loads((HERE / "manifest.json").read_text())
+        old.setdefault("window_cases", {}).update(write_window(gl, tws, FKC))
         (HERE / "manifest.json").write_text(json.dumps(old, indent=1) + "\n")
         return
 
@@ -233,7 +285,20 @@ def main():
     d = (rng.random((32, 32)) * 1e-38).astype(F32)
     f = (rng.random((32, 32, 4)) * 1e-39).astype(F32)
     cases.append(("denormal32", b, d, f, FKC, (1, 5), "closed", None))
+    # 6. the reference's own edge at a size its dispatch shapes truncate:
+    #    100 x 88 runs 6 x 5 work groups of 16^2 (columns >= 96 and rows >= 80
+    #    never change) and a 3 x 2 brush grid of 32^2 (rain only below row 64)
+    b, d, f = rnd(106, 88, 100)
+    cases.append(("random100x88_dispatch", b, d, f, FKC, (1, 25), "dispatch",
+                  (50.0, 70.0, 1e-2, 400.0)))
 
+    only = {c for c in args.cases.split(",") if c}
+    if only:
+        old = json.loads((HERE / "manifest.json").read_text())
+        manifest["cases"] = old["cases"]
+        manifest["mip_cases"] = old.get("mip_cases", {})
+        manifest["window_cases"] = old.get("window_cases", {})
+        cases = [c for c in cases if c[0] in only]
     for name, b, d, f, fkc, cps, boundary, rain in cases:
         t0 = time.time()
         out = run_case(gl, b, d, f, fkc, cps, boundary, rain)
@@ -251,7 +316,11 @@ def main():
                        for k, v in arrays.items()},
         }
         print(f"{name}: {time.time() - t0:.1f}s", flush=True)
+    if only:
+        (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1) + "\n")
+        return
     manifest["mip_cases"] = write_mips(gl, tws)
+    manifest["window_cases"] = write_window(gl, tws, FKC)
     (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1) + "\n")
 
     if args.skip_timing:

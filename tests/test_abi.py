@@ -86,3 +86,33 @@ def test_fbm_terrain_is_deterministic(tws):
     # multithreaded generation matches row-by-row generation
     rows = np.stack([tws.generate_fbm_terrain(256, 256, seed=3)[y] for y in (0, 100, 255)])
     assert np.array_equal(rows, big[[0, 100, 255]])
+
+
+def test_tune_record_validates_and_forget_clears(tws):
+    lib = tws.lib()
+    assert lib.tws_tune_record(0, 64, tws.KERNEL_TILE_128x64, 4) == -1
+    assert lib.tws_tune_record(64, 64, 99, 4) == -1
+    assert lib.tws_tune_record(64, 64, tws.KERNEL_TILE_128x64, 7) == -1
+    assert b"tws_tune_record" in lib.tws_last_error()
+    assert lib.tws_tune_record(64, 64, tws.KERNEL_TILE_128x64, 4) == 0
+    assert lib.tws_tune_forget() == 0
+
+
+def test_closed_walls_bound_k(tws):
+    """Closed walls need the wall ghost's head difference times K to stay
+    finite: K above 1e8 is refused with closed walls, accepted otherwise."""
+    import ctypes as C
+    h = C.c_void_p()
+    bad = tws.params_raw(0.99, 2e8, 0.02, tws.BOUNDARY_CLOSED)
+    assert tws.lib().tws_create(64, 64, C.byref(bad), None, C.byref(h)) == -1
+    assert not h.value
+    for bnd in (tws.BOUNDARY_ZERO, tws.BOUNDARY_REFERENCE_DISPATCH):
+        p = tws.params_physical(0.02, 9.81, 1.0, 1.0, 0.995, boundary=bnd)
+        assert p.boundary == bnd
+    with pytest.raises(tws.TwsError):
+        tws.params_physical(0.02, 9.81, 1.0, 1.0, 0.995, boundary=3)
+
+
+def test_stream_copy_shapes(tws):
+    assert tws.stream_copy_shapes() >= 4
+    assert tws.lib().tws_stream_copy_bench(1 << 20, 1, tws.stream_copy_shapes(), None) == -1

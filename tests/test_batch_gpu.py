@@ -179,10 +179,11 @@ def test_workload_grids_vs_restatements(gpu, oracle):
         assert err.max() < 1e-4 * m
 
 
-def test_autotune_records_the_fastest_candidate(gpu):
-    """tws_autotune times every (variant, steps per launch) candidate, returns
-    the fastest, and TWS_KERNEL_AUTO picks it for grids of that size class
-    until tws_tune_forget restores the built-in table."""
+def test_autotune_records_only_on_request(gpu):
+    """tws_autotune times every (variant, steps per launch) candidate and
+    returns the fastest without changing what TWS_KERNEL_AUTO picks; after
+    tws_tune_record, AUTO picks the recorded launch for grids of that size
+    class until tws_tune_forget restores the built-in table."""
     shapes = [(300, 280), (260, 310), (290, 290)]
     p = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995)
     with gpu.Simulation(290, 290, p) as s:
@@ -196,8 +197,33 @@ def test_autotune_records_the_fastest_candidate(gpu):
         top = max(trials, key=lambda t: t["cell_updates_per_s"])
         assert (best["kernel"], best["k"]) == (top["kernel"], top["k"])
         with gpu.Simulation(290, 290, p) as s:
-            assert s.get_launch() == (best["kernel"], best["k"])
+            assert s.get_launch() == builtin, "autotune alone changed the default launch"
+        other = next((kern, k) for kern in gpu.TILE_KERNELS for k in (2, 3)
+                     if (kern, k) != builtin)
+        gpu.tune_record(290, 290, *other)
+        with gpu.Simulation(290, 290, p) as s:
+            assert s.get_launch() == other
+        with gpu.Simulation(1024, 1024, p) as s:  # another size class
+            assert s.get_launch() != other or other == builtin
     finally:
         gpu.tune_forget()
     with gpu.Simulation(290, 290, p) as s:
         assert s.get_launch() == builtin
+
+
+def test_batch_refuses_truncating_dispatch_grids(gpu):
+    """A reference-dispatch grid whose sides are not multiples of 32 steps on
+    the two-pass kernels of the single-grid path; batches and servers refuse
+    it, and accept it at multiples of 32 (where it is the ZERO edge)."""
+    p = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995,
+                            boundary=gpu.BOUNDARY_REFERENCE_DISPATCH)
+    pool = 4 * gpu.grid_footprint(128, 128)
+    with gpu.Batch(4, pool) as bt:
+        with pytest.raises(gpu.TwsError):
+            bt.add(100, 88, p)
+        gid = bt.add(128, 96, p)
+        q = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995)
+        other = bt.add(100, 88, q)
+        with pytest.raises(gpu.TwsError):
+            bt.set_params(other, p)
+        assert gid != other

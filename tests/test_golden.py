@@ -46,7 +46,8 @@ def test_oracle_matches_reference_shaders(oracle, name):
     z = load(name)
     F, K, Cc = case["F_value"], case["K_value"], case["C_value"]
     assert f"0x{np.float32(F).view(np.uint32):08x}" == case["F"]
-    boundary = oracle.BOUNDARY_CLOSED if case["boundary"] == "closed" else oracle.BOUNDARY_ZERO
+    boundary = {"closed": oracle.BOUNDARY_CLOSED, "zero": oracle.BOUNDARY_ZERO,
+                "dispatch": oracle.BOUNDARY_DISPATCH}[case["boundary"]]
     for n in case["steps"]:
         d, f, v = oracle.step(z["b"], z["d0"], z.get("f0"), F, K, Cc, n=n, boundary=boundary,
                               rain=case["rain"], velocity=True)
@@ -73,3 +74,45 @@ def test_fixtures_are_not_trivial():
     assert np.count_nonzero(z["f_1000"]) > 1000
     z = load("random64x96_zero_edge")
     assert z["d_20"].sum(dtype=np.float64) < z["d0"].sum(dtype=np.float64)  # drained
+    # the reference's dispatch shapes leave the tail of a 100 x 88 grid as it was
+    z = load("random100x88_dispatch")
+    assert np.array_equal(z["d_25"][80:], z["d0"][80:])
+    assert np.array_equal(z["d_25"][:, 96:], z["d0"][:, 96:])
+    assert np.array_equal(z["f_25"][:, 96:], z["f0"][:, 96:])
+    assert not z["v16_25"][:, 96:].any()
+    assert not np.array_equal(z["d_25"][64:80, :96], z["d0"][64:80, :96])
+
+
+WINDOWS = sorted(MANIFEST.get("window_cases", {}))
+
+
+def test_window_case_present():
+    assert "config2_4096_window64" in WINDOWS
+
+
+@pytest.mark.parametrize("name", WINDOWS)
+def test_oracle_matches_reference_window(tws, oracle, name):
+    """The headline configuration (4096^2 fBm + 0.5 m sheet, closed walls)
+    after 64 steps of the reference shaders, centre window: the restatement
+    run on a sub-domain that holds the window's whole dependency cone (2 cells
+    per step) plus margin reproduces it bit for bit."""
+    case = MANIFEST["window_cases"][name]
+    z = load(name)
+    N = case["shape"][0]
+    (n,) = case["steps"]
+    y0, x0, wh, ww = case["window"]
+    b = tws.generate_fbm_terrain(N, N, seed=2, octaves=6, base_frequency=4, persistence=0.43,
+                                 height_scale=100.0)
+    import hashlib
+    assert hashlib.sha256(b.tobytes()).hexdigest() == case["terrain_sha256"]
+    m = 2 * n + 64
+    sub = (slice(y0 - m, y0 + wh + m), slice(x0 - m, x0 + ww + m))
+    bs = np.ascontiguousarray(b[sub])
+    d, f, v = oracle.step(bs, np.full(bs.shape, 0.5, F32), None, case["F_value"],
+                          case["K_value"], case["C_value"], n=n, velocity=True)
+    w = (slice(m, m + wh), slice(m, m + ww))
+    assert np.array_equal(bs[w], z["b_window"])
+    assert np.array_equal(d[w], z[f"d_{n}"])
+    assert np.array_equal(f[w], z[f"f_{n}"])
+    assert np.array_equal(half_rtz_numpy(v[w]), z[f"v16_{n}"])
+    assert z[f"f_{n}"].any() and not np.array_equal(z[f"d_{n}"], np.full((wh, ww), 0.5, F32))

@@ -236,3 +236,37 @@ def test_params_reference_defaults(oracle):
     assert K2 == float(F32(0.02 * 1.0 * float(F32(9.81)) / 1.0)) or abs(K2 - 0.1962) < 1e-7
     assert C2 == float(F32(float(F32(0.02))))
     assert abs(F2 - 0.995 ** 0.02) < 1e-7
+
+
+def test_dispatch_mode_freezes_the_uncovered_tail(oracle):
+    """BOUNDARY_DISPATCH: the reference's truncating dispatch shapes.  At
+    multiples of 32 it is the ZERO edge; otherwise columns >= 16*(W/16) and
+    rows >= 16*(H/16) keep depth and flux, and the rain reaches only
+    [0, 32*(W/32)) x [0, 32*(H/32))."""
+    rng = np.random.default_rng(5)
+    F, K, Cc = oracle.params_physical()
+    for H, W in ((64, 96), (70, 90), (40, 33)):
+        b = (rng.random((H, W)) * 4).astype(np.float32)
+        d = (rng.random((H, W)) * 2).astype(np.float32)
+        f = (rng.random((H, W, 4)) * 0.3).astype(np.float32)
+        rain = (W / 2.0, H - 3.0, 1e-2, 900.0)
+        dz, fz, _ = oracle.step(b, d, f, F, K, Cc, n=7, boundary=oracle.BOUNDARY_ZERO, rain=rain)
+        dd, fd, vd = oracle.step(b, d, f, F, K, Cc, n=7, boundary=oracle.BOUNDARY_DISPATCH,
+                                 rain=rain, velocity=True)
+        if H % 32 == 0 and W % 32 == 0:
+            assert np.array_equal(dz, dd) and np.array_equal(fz, fd)
+            continue
+        uh, uw = H // 16 * 16, W // 16 * 16
+        assert np.array_equal(dd[uh:], d[uh:]) and np.array_equal(dd[:, uw:], d[:, uw:])
+        assert np.array_equal(fd[uh:], f[uh:]) and np.array_equal(fd[:, uw:], f[:, uw:])
+        assert not vd[uh:].any() and not vd[:, uw:].any()
+        assert not np.array_equal(dd[:uh, :uw], d[:uh, :uw])
+        # the covered cells read the frozen tail as ordinary neighbours: a lake
+        # at rest stays at rest next to it, while the image edge at x = -1,
+        # y = -1 (read as 0) drains the lake there, 2 cells per step
+        flat = np.zeros((H, W), np.float32)
+        d1, f1, _ = oracle.step(flat, np.ones((H, W), np.float32), None, F, K, Cc, n=3,
+                                boundary=oracle.BOUNDARY_DISPATCH)
+        inner = (slice(8, uh), slice(8, uw))
+        assert not f1[inner].any() and (d1[inner] == 1.0).all()
+        assert f1[:2, :uw].any() and (d1[0, :uw] < 1.0).all()

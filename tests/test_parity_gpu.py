@@ -57,7 +57,8 @@ def test_lane_shift_layout(gpu):
 
 KERNELS = [(2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 4), (3, 5), (3, 8), (4, 1), (4, 2), (4, 4),
            (4, 5), (4, 8), (5, 1), (5, 4), (5, 5), (5, 6), (5, 8), (6, 1), (6, 3), (6, 5), (6, 8),
-           (7, 1), (7, 2), (7, 4), (7, 6), (7, 8)]
+           (7, 1), (7, 2), (7, 4), (7, 6), (7, 8), (8, 2), (8, 3), (8, 4), (8, 5), (8, 6),
+           (8, 8)]
 
 
 @pytest.mark.parametrize("kernel,k", KERNELS)
@@ -80,7 +81,7 @@ RAIN_DISCS = [(100.0, 60.0, 1e-3, 256.0),     # inside
               (64.25, 56.75, 5e-3, 3.0)]       # a few cells at a wave's first rows
 
 
-@pytest.mark.parametrize("kernel,k", [(3, 4), (3, 5), (5, 8), (2, 3), (4, 6)])
+@pytest.mark.parametrize("kernel,k", [(3, 4), (3, 5), (5, 8), (2, 3), (4, 6), (8, 5), (8, 3)])
 def test_rain_disc_positions(gpu, oracle, kernel, k):
     """The step kernels skip the brush for waves that the disc cannot reach; a
     disc at every kind of position (under a single cell, across tile seams,
@@ -104,7 +105,7 @@ def test_twopass_bitwise(gpu, oracle):
 
 
 @pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 2), (4, 8), (3, 5), (5, 4), (6, 8),
-                                      (7, 6), (7, 8)])
+                                      (7, 6), (7, 8), (8, 5)])
 def test_zero_boundary_bitwise(gpu, oracle, kernel, k):
     b, d, f = random_state(21, 130, 259)
     p = gpu.params_physical(**CFG_PARAMS, boundary=gpu.BOUNDARY_ZERO)
@@ -119,7 +120,7 @@ def test_ragged_and_tiny_grids(gpu, oracle, shape):
     H, W = shape
     b, d, f = random_state(H * 1000 + W, H, W)
     p = gpu.params_physical(**CFG_PARAMS)
-    for kernel, k in ((2, 4), (4, 8), (1, 1), (3, 5), (5, 6), (6, 8), (7, 8)):
+    for kernel, k in ((2, 4), (4, 8), (1, 1), (3, 5), (5, 6), (6, 8), (7, 8), (8, 5), (8, 8)):
         out = run_gpu(gpu, b, d, f, 6, kernel, k, p)
         de, fe, ve = oracle.step(b, d, f, *p3(p), n=6, velocity=True)
         assert_same(out, de, fe, ve)
@@ -314,3 +315,66 @@ def test_reference_default_scene_steps_bitwise(gpu, oracle):
                              velocity=True)
     assert_same(out, de, fe, ve)
     assert out["depth"].sum() > 0
+
+
+@pytest.mark.parametrize("shape,rain", [((88, 100), (50.0, 70.0, 1e-2, 400.0)),
+                                        ((168, 200), None),
+                                        ((33, 47), (10.0, 30.0, 1e-2, 100.0)),
+                                        ((96, 128), (60.0, 40.0, 1e-2, 300.0))])
+def test_reference_dispatch_bitwise(gpu, oracle, shape, rain):
+    """TWS_BOUNDARY_REFERENCE_DISPATCH: the reference's truncating dispatch
+    shapes (Terrain.cpp:167,258,264) on the default launch and two others,
+    then a one-shot brush: bit-identical to the restatement's DISPATCH mode.
+    At 96 x 128 (multiples of 32) the tile kernels run it as the ZERO edge."""
+    H, W = shape
+    b, d, f = random_state(H * 7 + W, H, W)
+    p = gpu.params_physical(**CFG_PARAMS, boundary=gpu.BOUNDARY_REFERENCE_DISPATCH)
+    de, fe, ve = oracle.step(b, d, f, *p3(p), n=9, boundary=oracle.BOUNDARY_DISPATCH, rain=rain,
+                             velocity=True)
+    for kernel, k in ((0, 0), (3, 5), (1, 1)):
+        out = run_gpu(gpu, b, d, f, 9, kernel, k, p, rain=rain)
+        assert_same(out, de, fe, ve)
+    with gpu.Simulation(W, H, p) as s:
+        s.upload(b, d, f)
+        s.apply_brush(W - 3.5, H - 2.25, 0.5, 64.0)
+        got = s.download(flux=False)["depth"]
+    bw, bh = W // 32 * 32, H // 32 * 32
+    want = d.copy()
+    if bw and bh:
+        want[:bh, :bw] = oracle.brush(np.ascontiguousarray(d[:bh, :bw]), W - 3.5, H - 2.25, 0.5,
+                                      64.0)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 5), (4, 8), (5, 6), (6, 3), (7, 8),
+                                      (8, 5)])
+@pytest.mark.parametrize("K", [0.0, 1e-35])
+def test_closed_walls_exact_for_any_k(gpu, oracle, kernel, k, K):
+    """Closed walls with K = 0 (and a K so small that the wall's head
+    difference no longer outweighs a stored flux), from a state whose edge
+    cells hold uploaded flux toward the walls: bit-identical to the
+    restatement, which zeroes every wall-ward flux, and no water is lost."""
+    b, d, f = random_state(31 + kernel, 70, 130)
+    p = gpu.params_raw(0.99, K, 0.02, gpu.BOUNDARY_CLOSED)
+    rain = (20.0, 60.0, 1e-2, 100.0)
+    out = run_gpu(gpu, b, d, f, 7, kernel, k, p, rain=rain)
+    de, fe, ve = oracle.step(b, d, f, 0.99, K, 0.02, n=7, rain=rain, velocity=True)
+    assert_same(out, de, fe, ve)
+    dep = float(oracle.brush(np.zeros((70, 130)), *rain, dtype=np.float64).sum()) * 7
+    v0 = float(d.sum(dtype=np.float64))
+    assert abs(float(out["depth"].sum(dtype=np.float64)) - (v0 + dep)) < 1e-5 * (v0 + dep)
+
+
+@pytest.mark.parametrize("k", [2, 4, 5, 6, 8])
+def test_stream_segments_bitwise(gpu, oracle, k):
+    """The streaming kernel cuts a tall grid into many strip segments, each
+    sweeping 2k rows into its neighbours; 13 steps (full launches and a tail
+    of other step counts) with rain across segment seams, bitwise against the
+    restatement."""
+    H, W = 1500, 300
+    b, d, f = random_state(400 + k, H, W)
+    p = gpu.params_physical(**CFG_PARAMS)
+    rain = (150.0, 700.0, 5e-3, 2500.0)
+    out = run_gpu(gpu, b, d, f, 13, gpu.KERNEL_STREAM_128, k, p, rain=rain)
+    de, fe, ve = oracle.step(b, d, f, *p3(p), n=13, rain=rain, velocity=True)
+    assert_same(out, de, fe, ve)

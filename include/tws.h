@@ -229,6 +229,22 @@ int tws_check_finite(tws_sim* sim);
 int tws_sync(tws_sim* sim);
 int tws_get_device_planes(const tws_sim* sim, tws_device_planes* out);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The FP64 volume sum(d) of a depth plane in device
+ * memory (`height` rows of `pitch` floats, the first `width` of each used),
+ * bit for bit what tws_volume returns for a handle's grid: the same kernel
+ * and fixed-order reduction, with its 512 partial sums in a fixed 4 KiB
+ * device array of the library (one per device, kept until the process ends)
+ * instead of a handle's buffer.  Needs no handle and allocates nothing.
+ * Runs on `stream` (NULL: the default stream) and synchronises it.
+ * TWS_ERR_INVALID for NULL pointers or a plane beyond the limits (width and
+ * height 1..2^20, width <= pitch <= 2^30).  Every call on a device writes all
+ * 512 partials before reading them, so nothing carries over from the last
+ * call; calls that overlap on one device (other threads or streams) share
+ * the array and race: serialise them. */
+int tws_volume_noalloc(const float* depth, int32_t width, int32_t height, int64_t pitch,
+                       void* stream, double* out);
 
 /* ---- launch configuration ------------------------------------------------ */
 

@@ -189,6 +189,7 @@ PROTOTYPES = {
     "tws_batch_sync": (C.c_int, [_vp]),
     "tws_autotune": (C.c_int, [_vp, _vp, _i32, _i32, C.POINTER(TuneEntry), C.POINTER(TuneEntry), _i32, C.POINTER(_i32)]),
     "tws_tune_forget": (C.c_int, []),
+    "tws_volume_noalloc": (C.c_int, [_vp, _i32, _i32, C.c_int64, _vp, C.POINTER(C.c_double)]),
     "tws_tune_record": (C.c_int, [_i32, _i32, _i32, _i32]),
     "tws_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "tws_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),

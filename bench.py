@@ -47,6 +47,7 @@ TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
 LLVMPIPE_FILE = ROOT / "profiles" / "cpu_baseline_llvmpipe.json"
 CU_COUNT = 256                 # MI355X compute units
 CLOCK_HZ = 2.4e9               # MI355X peak engine clock
+KERNEL_STREAM = 8              # TWS_KERNEL_STREAM_128
 # one wave64 VALU instruction per cycle per CU (4 SIMDs, 16 lanes each)
 VALU_WAVE_INSTS_PER_S = CU_COUNT * CLOCK_HZ
 
@@ -140,7 +141,7 @@ def roofline(kernel, k, N, mean_ms, copy_gbps):
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_BPS,
         "traffic": None,
-        "kernel": "k_tile",
+        "kernel": "k_stream" if kernel == KERNEL_STREAM else "k_tile",
         "steps_per_launch": k,
         "mean_launch_ms": mean_ms,
         "compulsory_bytes_per_launch": compulsory,

@@ -29,6 +29,10 @@ int device_cus();
 // Launch configuration (kernel variant, steps per launch) for a grid size,
 // from the tuning table in tws_tune.cpp.
 void choose_launch(int width, int height, int* kernel, int* k);
+// The best tile-kernel launch for a grid size: choose_launch's choice for
+// grids it does not give the streaming kernel, and the fallback for grids
+// the streaming kernel cannot step.
+void tile_launch(int width, int height, int* kernel, int* k);
 // Tile order (columns per group) for a grid's default launch.
 int default_tile_group(int width, int height, int kernel, int k);
 

@@ -35,6 +35,13 @@ struct tws_sim {
 
 namespace tws {
 
+// Whether the streaming kernel can step this grid: closed walls, and each
+// plane smaller than 2 GiB (its stores address a plane with 31-bit offsets
+// and use offset 2^31 for lanes that must not write).
+static bool stream_usable(const GridState& gs) {
+    return gs.params.boundary == TWS_BOUNDARY_CLOSED && gs.pitch * gs.H < ((int64_t)1 << 29);
+}
+
 // Streaming launches (tws_stream.hip): full launches of k steps, then the
 // remainder as launches of the largest built step counts that fit it (each
 // launch runs exactly its template's number of levels).
@@ -78,10 +85,11 @@ int step_grid(GridState& gs, int kernel, int k, int32_t n, hipStream_t stream, h
         return hipEventRecord(ev[2 * nl + which], stream);
     };
     if (kernel == TWS_KERNEL_STREAM_128) {
-        // closed walls only; other edges take the large-grid tile kernel
-        if (gs.params.boundary == TWS_BOUNDARY_CLOSED)
+        // closed walls and planes under 2 GiB (its stores address a plane
+        // with 31-bit offsets); other grids take the tile kernel's launch
+        if (stream_usable(gs))
             return step_stream(gs, k, n, stream, ev, ev_pairs, launches);
-        kernel = TWS_KERNEL_TILE_128x64;
+        tile_launch(gs.W, gs.H, &kernel, &k);
     }
     if (kernel == TWS_KERNEL_TWOPASS || needs_twopass(gs.desc())) {
         for (int i = 0; i < n; ++i) {

@@ -14,8 +14,8 @@
 //   - no work-group barrier: the wavefronts of a work group are independent,
 //     x-neighbours come from DPP lane shifts, y-neighbours from the lane's
 //     own window;
-//   - terrain b, read by each level at its own row, sits in a wave-private
-//     ring of 16 rows in LDS (and the rain amounts beside it);
+//   - terrain b, read by each level at its own rows, sits in a wave-private
+//     ring of 16 (32 for 8 levels) rows in LDS, the rain amounts beside it;
 //   - the next input row is loaded one row-step ahead.
 // The x-halo is the tile kernels': HALOX = 2 KH columns on each side go stale
 // and are recomputed by the neighbouring strip.  Out-of-domain cells are the
@@ -24,6 +24,7 @@
 // to the FP32 restatement for any segment length and steps per launch.
 
 #include <algorithm>
+#include <type_traits>
 
 #include "tws.h"
 #include "tws_internal.hpp"
@@ -35,73 +36,137 @@ namespace {
 
 constexpr int kSCPL = 2;                 // columns per lane
 constexpr int kSEXW = kSCPL * kWave;     // strip width incl. halo
-constexpr int kRing = 16;                // rows of b (and rain) kept per wave
 constexpr int kStreamWaves = 4;          // independent wavefronts per work group
 
 template <int KH>
 struct StreamGeom {
-    // Wavefronts per SIMD: a level's window takes about 36 VGPRs, so up to 5
-    // levels fit 256 VGPRs (2 per SIMD); 6 and 8 levels take 1 per SIMD
-    // with up to 512 VGPRs instead of spilling.
-    static constexpr int WAVES_PER_SIMD = KH <= 5 ? 2 : 1;
+    // Wavefronts per SIMD: a level's window takes about 32 VGPRs, so up to 4
+    // levels fit 168 VGPRs (3 per SIMD), 6 levels 256 (2 per SIMD); 8 levels
+    // take 1 per SIMD with up to 512 VGPRs instead of spilling.
+    static constexpr int WAVES_PER_SIMD = KH <= 4 ? 3 : (KH <= 6 ? 2 : 1);
     static constexpr int HALOX = ((2 * KH + kSCPL - 1) / kSCPL) * kSCPL;
     static constexpr int OX = kSEXW - 2 * HALOX;
-    static_assert(2 * KH - 1 <= kRing, "ring too short for KH");
+    // rows of b (and rain) kept per wave: the levels read rows t-2KH+1 ..
+    // t+1 at row-step t
+    static constexpr int RING = 2 * KH + 1 <= 16 ? 16 : 32;
     static_assert(OX > 0, "halo wider than the strip");
 };
 
-// One level's window: flux of rows a-1 (F1) and a-2 (F2), depth and head of
-// rows a-2 (D1, H1) and a-3 (D2, H2), a being the level's flux row at the
-// current row-step.  Depth and head already include the next level's rain.
-struct Level {
-    float F1[4][kSCPL], F2[4][kSCPL];
-    float D1[kSCPL], D2[kSCPL], H1[kSCPL], H2[kSCPL];
-};
+// Index of a ring slot: a value of row r lives in slot (r mod n) of its ring.
+// Rows are counted by their phase relative to the first row-step, so that
+// the six row-steps of the unrolled loop body use compile-time slots.
+__device__ constexpr int slot_of(int phase, int n) { return ((phase % n) + n) % n; }
 
-// The six planes of one row for this lane's columns.
-struct Row {
-    float b[kSCPL], d[kSCPL], f[4][kSCPL];
-};
+// Row loads are unconditional, from an address clamped into the grid's
+// rows (x0 is even and at most pitch - 2, so both columns lie in the row's
+// pitch), so no branch surrounds them and the loads of later rows stay in
+// flight while a row-step computes.  Values of cells outside the domain are
+// replaced where they are consumed (bd_fix, f_fix).
+// The row part of an address is wave-uniform (scalar registers) and the
+// column part a per-lane constant `xc` (x0 clamped), so a load needs no
+// per-lane address arithmetic.
+__device__ __forceinline__ int64_t row_base(const GridDesc& g, int y) {
+    return (int64_t)min(max(y, 0), g.H - 1) * g.pitch;
+}
 
-}  // namespace
+// A lane's element of a row: the row's address, made a scalar (it is the
+// same in every lane), plus the lane's byte offset zero-extended from 32
+// bits, so the load takes its base from scalar registers and needs no
+// per-lane 64-bit address.
+__device__ __forceinline__ const gfloat* at(const float* plane, int64_t row, uint32_t xbytes) {
+    const uint64_t a = reinterpret_cast<uint64_t>(plane + row);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    const char __attribute__((address_space(1)))* p =
+        (const char __attribute__((address_space(1)))*)(((uint64_t)hi << 32) | lo);
+    return (const gfloat*)(p + xbytes);
+}
 
-// Loads row y of set `parity` for columns x0..x0+kSCPL-1 (ghost values
-// outside the domain).  Closed walls: an edge cell's flux toward the wall is
-// dropped (the walls carry nothing, whatever was uploaded).
-__device__ __forceinline__ void stream_load(const GridDesc& g, int parity, int y, int x0, Row& r) {
-    const bool vin = (unsigned)y < (unsigned)g.H && x0 >= 0 && x0 < g.W;
-    float vb[kSCPL], vd[kSCPL], v0[kSCPL], v1[kSCPL], v2[kSCPL], v3[kSCPL];
+__device__ __forceinline__ void load_bd(const GridDesc& g, int parity, int y, uint32_t xb,
+                                        float (&b)[kSCPL], float (&d)[kSCPL]) {
+    const int64_t r = row_base(g, y);
+    vload<kSCPL>(at(g.b, r, xb), b);
+    vload<kSCPL>(at(g.d[parity], r, xb), d);
+}
+
+__device__ __forceinline__ void load_f(const GridDesc& g, int parity, int y, uint32_t xb,
+                                       float (&f)[4][kSCPL]) {
+    const int64_t r = row_base(g, y);
 #pragma unroll
-    for (int c = 0; c < kSCPL; ++c) vb[c] = vd[c] = v0[c] = v1[c] = v2[c] = v3[c] = 0.0f;
-    if (vin) {
-        const int64_t off = (int64_t)y * g.pitch + x0;
-        vload<kSCPL>(gptr(g.b) + off, vb);
-        vload<kSCPL>(gptr(g.d[parity]) + off, vd);
-        vload<kSCPL>(gptr(g.f[parity][0]) + off, v0);
-        vload<kSCPL>(gptr(g.f[parity][1]) + off, v1);
-        vload<kSCPL>(gptr(g.f[parity][2]) + off, v2);
-        vload<kSCPL>(gptr(g.f[parity][3]) + off, v3);
-    }
+    for (int ch = 0; ch < 4; ++ch) vload<kSCPL>(at(g.f[parity][ch], r, xb), f[ch]);
+}
+
+// Stores of the new state are buffer stores that always issue: a lane whose
+// cell is not this wavefront's to write gives an offset past the plane's end
+// and the hardware drops the write.  With no branch around them, the waits
+// on the loads in flight (which share the memory counter with the stores)
+// can count the stores, instead of assuming they were skipped and waiting
+// for loads issued one row-step ago.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kNoStore = 0x80000000u;  // past every plane the kernel is given
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(float* plane, uint32_t bytes) {
+    // raw buffer (stride 0); the last descriptor word as for all gfx9 raw buffers
+    return __builtin_amdgcn_make_buffer_rsrc(plane, (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ void store2(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b) {
+    // cache policy 2: non-temporal (no tile of the launch reads it back)
+    __builtin_amdgcn_raw_buffer_store_b64(u32x2{__float_as_uint(a), __float_as_uint(b)}, r,
+                                          (int)off, 0, 2);
+}
+
+// Ghost terrain and depth for cells of row y outside the domain (the
+// closed-wall ghost b = 1e30, d = 0).  `inner` (wave-uniform): every column
+// of the strip lies inside the domain.
+__device__ __forceinline__ void bd_fix(const GridDesc& g, int y, int x0, bool inner,
+                                       float (&b)[kSCPL], float (&d)[kSCPL]) {
+    if (inner && (unsigned)y < (unsigned)g.H) return;
 #pragma unroll
     for (int c = 0; c < kSCPL; ++c) {
-        const bool in = vin && (x0 + c < g.W);
-        r.b[c] = in ? vb[c] : g.ghost_b;
-        r.d[c] = in ? vd[c] : g.ghost_d;
-        r.f[0][c] = in && x0 + c != g.W - 1 ? v0[c] : 0.0f;
-        r.f[1][c] = in && x0 + c != 0 ? v1[c] : 0.0f;
-        r.f[2][c] = in && y != g.H - 1 ? v2[c] : 0.0f;
-        r.f[3][c] = in && y != 0 ? v3[c] : 0.0f;
+        const bool in = (unsigned)y < (unsigned)g.H && (unsigned)(x0 + c) < (unsigned)g.W;
+        b[c] = in ? b[c] : g.ghost_b;
+        d[c] = in ? d[c] : g.ghost_d;
     }
 }
+
+// Zero flux outside the domain, and no flux toward a closed wall from the
+// cells along it (the walls carry nothing, whatever was uploaded).  `inner`
+// (wave-uniform): no column of the strip lies outside or on the first or
+// last column of the domain.
+__device__ __forceinline__ void f_fix(const GridDesc& g, int y, int x0, bool inner,
+                                      float (&f)[4][kSCPL]) {
+    if (inner && y >= 1 && y < g.H - 1) return;
+#pragma unroll
+    for (int c = 0; c < kSCPL; ++c) {
+        const int x = x0 + c;
+        const bool in = (unsigned)y < (unsigned)g.H && (unsigned)x < (unsigned)g.W;
+        f[0][c] = in && x != g.W - 1 ? f[0][c] : 0.0f;
+        f[1][c] = in && x != 0 ? f[1][c] : 0.0f;
+        f[2][c] = in && y != g.H - 1 ? f[2][c] : 0.0f;
+        f[3][c] = in && y != 0 ? f[3][c] : 0.0f;
+    }
+}
+
+}  // namespace
 
 // k_stream: grid of ceil(tasks / kStreamWaves) work groups; task = segment *
 // strips + strip, segments of seg_rows output rows.  Closed walls only
 // (MODE kModePlain or kModeRain).
+//
+// Registers, per lane and column: level j >= 1 keeps the flux of its rows
+// a, a-1, a-2 in a 3-slot ring FS[j] (the new row a overwrites a-3), levels
+// 0..KH-1 the depth of their last three output rows in DS[j] (heads are
+// recomputed from the b ring as the next level reads them: 6 VGPRs per level
+// fewer, which with the rest lets 4 levels run 3 wavefronts per SIMD); the
+// input rows come through PB (b and d, two rows ahead) and PF (flux, the row
+// level 1 reads plus one ahead).
 template <int KH, int MODE>
 __global__ void __launch_bounds__(kStreamWaves * kWave, StreamGeom<KH>::WAVES_PER_SIMD)
 k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
     constexpr bool RAIN = MODE == kModeRain;
     using G = StreamGeom<KH>;
+    constexpr int kRing = G::RING;
     __shared__ float s_b[kStreamWaves][kRing][kSEXW];
     __shared__ float s_wet[RAIN ? kStreamWaves : 1][RAIN ? kRing : 1][kSEXW];
 
@@ -123,29 +188,39 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
     const int ox = strip * G::OX - G::HALOX;
     const int x0 = ox + lane * kSCPL;
     const int lx = lane * kSCPL;
+    // x0 is even and at most pitch - 2: both columns lie in the row's pitch
+    const uint32_t xc = (uint32_t)min(max(x0, 0), (int)g.pitch - kSCPL) * 4u;  // bytes
     const float F = g.F, K = g.K, C = g.C;
     const int wr = RAIN ? wave : 0;
+    // every column of the strip inside the domain (b, d), and none on its
+    // first or last column either (f)
+    const bool inner_bd = ox >= 0 && ox + kSEXW <= g.W;
+    const bool inner_f = ox >= 1 && ox + kSEXW <= g.W - 1;
 
-    // stored columns of this lane
+    // stored columns of this lane (both, or only the first one when the grid
+    // ends between them: the second store then lands in the row padding,
+    // which nothing reads)
     const bool col_ok = lx >= G::HALOX && lx < G::HALOX + G::OX && x0 < g.W;
-    const bool full = x0 + kSCPL <= g.W;
-    gfloat* const pd = gptr(g.d[parity ^ 1]);
-    gfloat* const p0 = gptr(g.f[parity ^ 1][0]);
-    gfloat* const p1 = gptr(g.f[parity ^ 1][1]);
-    gfloat* const p2 = gptr(g.f[parity ^ 1][2]);
-    gfloat* const p3 = gptr(g.f[parity ^ 1][3]);
+    const bool second_in = x0 + 1 < g.W;
+    const uint32_t plane_bytes = (uint32_t)(g.pitch * g.H * 4);  // < 2^31 (stream_usable)
+    const __amdgpu_buffer_rsrc_t rd = plane_rsrc(g.d[parity ^ 1], plane_bytes);
+    const __amdgpu_buffer_rsrc_t r0 = plane_rsrc(g.f[parity ^ 1][0], plane_bytes);
+    const __amdgpu_buffer_rsrc_t r1 = plane_rsrc(g.f[parity ^ 1][1], plane_bytes);
+    const __amdgpu_buffer_rsrc_t r2 = plane_rsrc(g.f[parity ^ 1][2], plane_bytes);
+    const __amdgpu_buffer_rsrc_t r3 = plane_rsrc(g.f[parity ^ 1][3], plane_bytes);
     bool bad = false;
 
-    // rows of the ring that the rain reaches (bit r & 15), wave-uniform
+    // rows of the b ring that the rain reaches (bit r & (kRing - 1)), wave-uniform
     unsigned rain_rows = 0;
     const bool rain_on = RAIN && g.rain_on != 0;
 
-    // Accepts input row y: writes b (and the rain amount) into the ring,
-    // returns d with the first level's rain added and its head.
-    auto accept = [&](int y, const Row& r, float (&dn)[kSCPL], float (&hn)[kSCPL]) {
+    // Accepts input row y: writes b (and the rain amount) into the ring and
+    // returns d with the first level's rain added.
+    auto accept = [&](int y, float (&b)[kSCPL], float (&d)[kSCPL], float (&dn)[kSCPL]) {
+        bd_fix(g, y, x0, inner_bd, b, d);
         const int slot = y & (kRing - 1);
 #pragma unroll
-        for (int c = 0; c < kSCPL; ++c) s_b[wave][slot][lx + c] = r.b[c];
+        for (int c = 0; c < kSCPL; ++c) s_b[wave][slot][lx + c] = b[c];
         bool wet_row = false;
         if (rain_on) {
             wet_row = __builtin_amdgcn_readfirstlane(
@@ -155,7 +230,7 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
         }
         rain_rows = (rain_rows & ~(1u << slot)) | ((wet_row ? 1u : 0u) << slot);
 #pragma unroll
-        for (int c = 0; c < kSCPL; ++c) dn[c] = r.d[c];
+        for (int c = 0; c < kSCPL; ++c) dn[c] = d[c];
         if (wet_row) {
 #pragma unroll
             for (int c = 0; c < kSCPL; ++c) {
@@ -167,80 +242,81 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
                 if (in) dn[c] = dn[c] + w;
             }
         }
-#pragma unroll
-        for (int c = 0; c < kSCPL; ++c) hn[c] = dn[c] + r.b[c];
     };
 
-    // ---- state at the first row-step -------------------------------------
     const int t_begin = Y0 - 2 * KH + 1;
     const int t_end = Y1 + 2 * KH - 2;
 #pragma unroll
     for (int s = 0; s < kRing; ++s)
 #pragma unroll
         for (int c = 0; c < kSCPL; ++c) s_b[wave][s][lx + c] = 0.0f;
-    Level L[KH + 1];
+    float FS[KH + 1][3][4][kSCPL];
+    float DS[KH][3][kSCPL];
+    float PB[2][2][kSCPL];   // [slot][b, d][column]
+    float PF[2][4][kSCPL];   // [slot][channel][column]
 #pragma unroll
     for (int j = 0; j <= KH; ++j)
 #pragma unroll
-        for (int c = 0; c < kSCPL; ++c) {
+        for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int ch = 0; ch < 4; ++ch) L[j].F1[ch][c] = L[j].F2[ch][c] = 0.0f;
-            L[j].D1[c] = L[j].D2[c] = L[j].H1[c] = L[j].H2[c] = 0.0f;
-        }
-    float Fown0[4][kSCPL];  // input flux of row t (the first level's own flux)
+            for (int c = 0; c < kSCPL; ++c) {
+#pragma unroll
+                for (int ch = 0; ch < 4; ++ch) FS[j][r][ch][c] = 0.0f;
+                if (j < KH) DS[j][r][c] = 0.0f;
+            }
+    // Rows relative to row-step t = t_begin (phase 0): level 0 holds rows
+    // t-1 (slot 2) and t (slot 0); PF rows t, t+1, t+2 (slots 0, 1, 2); PB rows
+    // t+1 (slot 1) and t+2 (slot 0).
     {
-        Row r;
-        stream_load(g, parity, t_begin - 1, x0, r);
-        accept(t_begin - 1, r, L[0].D2, L[0].H2);
-        stream_load(g, parity, t_begin, x0, r);
-        accept(t_begin, r, L[0].D1, L[0].H1);
-#pragma unroll
-        for (int ch = 0; ch < 4; ++ch)
-#pragma unroll
-            for (int c = 0; c < kSCPL; ++c) Fown0[ch][c] = r.f[ch][c];
+        float b[kSCPL], d[kSCPL];
+        load_bd(g, parity, t_begin - 1, xc, b, d);
+        accept(t_begin - 1, b, d, DS[0][2]);
+        load_bd(g, parity, t_begin, xc, b, d);
+        accept(t_begin, b, d, DS[0][0]);
     }
-    Row next;
-    stream_load(g, parity, t_begin + 1, x0, next);
+    load_f(g, parity, t_begin, xc, PF[0]);
+    load_f(g, parity, t_begin + 1, xc, PF[1]);
+    load_bd(g, parity, t_begin + 1, xc, PB[1][0], PB[1][1]);
+    load_bd(g, parity, t_begin + 2, xc, PB[0][0], PB[0][1]);
 
-    // ---- one row-step ----------------------------------------------------
-    auto step = [&](int t) {
-        // level 0: input row t+1, loaded one row-step ago; load row t+2
-        float dN0[kSCPL], hN0[kSCPL], fN0[4][kSCPL];
-        accept(t + 1, next, dN0, hN0);
-#pragma unroll
-        for (int ch = 0; ch < 4; ++ch)
-#pragma unroll
-            for (int c = 0; c < kSCPL; ++c) fN0[ch][c] = next.f[ch][c];
-        stream_load(g, parity, t + 2, x0, next);
+    // ---- one row-step at phase P (t = t_begin + P mod 6) ------------------
+    auto step = [&](auto phase, int t) {
+        constexpr int P = decltype(phase)::value;
+        // level 0: input row t+1, then the loads of row t+3
+        accept(t + 1, PB[slot_of(P + 1, 2)][0], PB[slot_of(P + 1, 2)][1],
+               DS[0][slot_of(P + 1, 3)]);
+        load_bd(g, parity, t + 3, xc, PB[slot_of(P + 1, 2)][0], PB[slot_of(P + 1, 2)][1]);
+        // input flux of row t, for level 1
+        f_fix(g, t, x0, inner_f, PF[slot_of(P, 2)]);
 
-        // inputs of level 1: level 0 at rows t-1, t, t+1
-        float hm1[kSCPL], h0[kSCPL], hp1[kSCPL], dOwn[kSCPL], dBase[kSCPL], fOwn[4][kSCPL];
-#pragma unroll
-        for (int c = 0; c < kSCPL; ++c) {
-            hm1[c] = L[0].H2[c];
-            h0[c] = L[0].H1[c];
-            hp1[c] = hN0[c];
-            dOwn[c] = L[0].D1[c];
-            dBase[c] = L[0].D2[c];
-#pragma unroll
-            for (int ch = 0; ch < 4; ++ch) fOwn[ch][c] = Fown0[ch][c];
-        }
-#pragma unroll
-        for (int c = 0; c < kSCPL; ++c) {
-            L[0].D2[c] = L[0].D1[c];
-            L[0].D1[c] = dN0[c];
-            L[0].H2[c] = L[0].H1[c];
-            L[0].H1[c] = hN0[c];
-#pragma unroll
-            for (int ch = 0; ch < 4; ++ch) Fown0[ch][c] = fN0[ch][c];
-        }
-
+        float dLast[kSCPL] = {0.0f, 0.0f};  // the last level's new depth row
 #pragma unroll
         for (int j = 1; j <= KH; ++j) {
-            const int a = t - 2 * (j - 1);  // flux row of level j
-            Level& Lj = L[j];
+            const int xj = P - 2 * (j - 1);          // phase of the level's flux row a
+            const int a = t - 2 * (j - 1);
+            // level j-1 at rows a-1 (m), a (o), a+1 (p); its own flux of row a
+            const int m = slot_of(xj + 2 - 3, 3), o = slot_of(xj + 2 - 2, 3), pp = slot_of(xj + 2 - 1, 3);
+            // heads of level j-1 at rows a-1, a, a+1: its depth plus b
+            float hm1[kSCPL], h0[kSCPL], hp1[kSCPL];
+            {
+                const int rm = (a - 1) & (kRing - 1), ro = a & (kRing - 1), rp = (a + 1) & (kRing - 1);
+#pragma unroll
+                for (int c = 0; c < kSCPL; ++c) {
+                    hm1[c] = DS[j - 1][m][c] + s_b[wave][rm][lx + c];
+                    h0[c] = DS[j - 1][o][c] + s_b[wave][ro][lx + c];
+                    hp1[c] = DS[j - 1][pp][c] + s_b[wave][rp][lx + c];
+                }
+            }
+            const float (&dOwn)[kSCPL] = DS[j - 1][o];
+            const float (&dBase)[kSCPL] = DS[j - 1][m];
+            const float (&fOwn)[4][kSCPL] = j == 1 ? PF[slot_of(P, 2)] : FS[j - 1][slot_of(xj + 2 - 2, 3)];
+            float (&FN)[4][kSCPL] = FS[j][slot_of(xj, 3)];
+            const float (&F1)[4][kSCPL] = FS[j][slot_of(xj - 1, 3)];
+            const float (&F2)[4][kSCPL] = FS[j][slot_of(xj - 2, 3)];
+            // levels start one after the other: level j's first row-step
+            // that can reach the output is t_begin + 4 (j - 1)
+            if (j > 1 && t < t_begin + 4 * (j - 1)) continue;
             // flux of row a: flowUpdate.comp:34-62
-            float FN[4][kSCPL];
             {
                 const float h_left = from_left_lane_or(h0[kSCPL - 1], h0[0]);
                 const float h_right = from_right_lane_or(h0[0], h0[kSCPL - 1]);
@@ -249,111 +325,92 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
                     const float own = h0[c];
                     const float hx1 = c < kSCPL - 1 ? h0[c + 1] : h_right;
                     const float hx0 = c > 0 ? h0[c - 1] : h_left;
-#pragma unroll
-                    for (int ch = 0; ch < 4; ++ch) FN[ch][c] = fOwn[ch][c];
+                    float o0 = fOwn[0][c], o1 = fOwn[1][c], o2 = fOwn[2][c], o3 = fOwn[3][c];
                     flux_update(F, K, C, dOwn[c], own - hx1, own - hx0, own - hp1[c],
-                                own - hm1[c], FN[0][c], FN[1][c], FN[2][c], FN[3][c]);
+                                own - hm1[c], o0, o1, o2, o3);
+                    FN[0][c] = o0; FN[1][c] = o1; FN[2][c] = o2; FN[3][c] = o3;
                 }
             }
+            if (j == 1) load_f(g, parity, t + 2, xc, PF[slot_of(P, 2)]);
             // depth of row a-1: flowApply.comp:32-41
-            float dN[kSCPL], hN[kSCPL];
+            float dN[kSCPL];
             {
-                const float in_left = from_left_lane(Lj.F1[0][kSCPL - 1]);   // X0.x
-                const float in_right = from_right_lane(Lj.F1[1][0]);        // X1.y
+                const float in_left = from_left_lane(F1[0][kSCPL - 1]);   // X0.x
+                const float in_right = from_right_lane(F1[1][0]);        // X1.y
 #pragma unroll
                 for (int c = 0; c < kSCPL; ++c) {
-                    const float X1y = c < kSCPL - 1 ? Lj.F1[1][c + 1] : in_right;
-                    const float X0x = c > 0 ? Lj.F1[0][c - 1] : in_left;
+                    const float X1y = c < kSCPL - 1 ? F1[1][c + 1] : in_right;
+                    const float X0x = c > 0 ? F1[0][c - 1] : in_left;
                     const float Y1w = FN[3][c];
-                    const float Y0z = Lj.F2[2][c];
+                    const float Y0z = F2[2][c];
                     const float in = ((X1y + X0x) + Y1w) + Y0z;
-                    const float out = ((Lj.F1[0][c] + Lj.F1[1][c]) + Lj.F1[2][c]) + Lj.F1[3][c];
+                    const float out = ((F1[0][c] + F1[1][c]) + F1[2][c]) + F1[3][c];
                     dN[c] = fmaxf(0.0f, dBase[c] + (in - out) * C);
                 }
             }
             const int yd = a - 1;  // row of the new depth
             if (j < KH) {
                 // the next level's rain, then its head
-                const int slot = yd & (kRing - 1);
-                if (RAIN && ((rain_rows >> slot) & 1u)) {
+                const int ring = yd & (kRing - 1);
+                if (RAIN && ((rain_rows >> ring) & 1u)) {
 #pragma unroll
                     for (int c = 0; c < kSCPL; ++c) {
                         const bool in = x0 + c >= 0 && x0 + c < g.W;
-                        if (in) dN[c] = dN[c] + s_wet[wr][slot][lx + c];
+                        if (in) dN[c] = dN[c] + s_wet[wr][ring][lx + c];
                     }
                 }
+                const int ds = slot_of(xj - 1, 3);
 #pragma unroll
-                for (int c = 0; c < kSCPL; ++c) hN[c] = dN[c] + s_b[wave][slot][lx + c];
-                // inputs of level j+1 (level j at rows a-3, a-2, a-1), then
-                // the window moves down one row
+                for (int c = 0; c < kSCPL; ++c) DS[j][ds][c] = dN[c];
+            } else {
 #pragma unroll
-                for (int c = 0; c < kSCPL; ++c) {
-                    hm1[c] = Lj.H2[c];
-                    h0[c] = Lj.H1[c];
-                    hp1[c] = hN[c];
-                    dOwn[c] = Lj.D1[c];
-                    dBase[c] = Lj.D2[c];
-#pragma unroll
-                    for (int ch = 0; ch < 4; ++ch) fOwn[ch][c] = Lj.F2[ch][c];
-                }
-#pragma unroll
-                for (int c = 0; c < kSCPL; ++c) {
-                    Lj.D2[c] = Lj.D1[c];
-                    Lj.D1[c] = dN[c];
-                    Lj.H2[c] = Lj.H1[c];
-                    Lj.H1[c] = hN[c];
-                }
-            } else if (yd >= Y0 && yd < Y1 && col_ok) {
-                // last level: store row a-1 (its flux is F1, final since the
-                // previous row-step)
-                const int64_t off = (int64_t)yd * g.pitch + x0;
-#pragma unroll
-                for (int c = 0; c < kSCPL; ++c)
-                    bad |= (full || x0 + c < g.W) &
-                           nonfinite5(dN[c], Lj.F1[0][c], Lj.F1[1][c], Lj.F1[2][c], Lj.F1[3][c]);
-                if (full) {
-                    vstore<kSCPL>(pd + off, dN);
-                    vstore<kSCPL>(p0 + off, Lj.F1[0]);
-                    vstore<kSCPL>(p1 + off, Lj.F1[1]);
-                    vstore<kSCPL>(p2 + off, Lj.F1[2]);
-                    vstore<kSCPL>(p3 + off, Lj.F1[3]);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < kSCPL; ++c)
-                        if (x0 + c < g.W) {
-                            pd[off + c] = dN[c];
-                            p0[off + c] = Lj.F1[0][c];
-                            p1[off + c] = Lj.F1[1][c];
-                            p2[off + c] = Lj.F1[2][c];
-                            p3[off + c] = Lj.F1[3][c];
-                        }
-                }
+                for (int c = 0; c < kSCPL; ++c) dLast[c] = dN[c];
             }
-#pragma unroll
-            for (int ch = 0; ch < 4; ++ch)
-#pragma unroll
-                for (int c = 0; c < kSCPL; ++c) {
-                    Lj.F2[ch][c] = Lj.F1[ch][c];
-                    Lj.F1[ch][c] = FN[ch][c];
-                }
+        }
+        // Last level: store its depth row a-1 = t - 2 KH + 1 and the flux
+        // of that row (F1, final since the previous row-step).  Outside the
+        // ramp-up predicate above, so the store always issues; lanes outside
+        // the segment (every lane during the ramp) or the strip's output
+        // columns store past the plane, which the hardware drops.
+        {
+            const int yd = t - 2 * KH + 1;
+            const float (&F1)[4][kSCPL] = FS[KH][slot_of(P - 2 * (KH - 1) - 1, 3)];
+            const bool mine = col_ok && yd >= Y0 && yd < Y1;
+            const uint32_t off = mine ? (uint32_t)((int64_t)yd * g.pitch + x0) * 4u : kNoStore;
+            bad |= mine && (nonfinite5(dLast[0], F1[0][0], F1[1][0], F1[2][0], F1[3][0]) ||
+                            (second_in &&
+                             nonfinite5(dLast[1], F1[0][1], F1[1][1], F1[2][1], F1[3][1])));
+            store2(rd, off, dLast[0], dLast[1]);
+            store2(r0, off, F1[0][0], F1[0][1]);
+            store2(r1, off, F1[1][0], F1[1][1]);
+            store2(r2, off, F1[2][0], F1[2][1]);
+            store2(r3, off, F1[3][0], F1[3][1]);
         }
     };
 
-    // Six row-steps per iteration: every window rotates with period 2 or 3,
-    // so the unrolled body renames registers instead of moving them.
-    for (int t = t_begin; t <= t_end; t += 6) {
-        step(t);
-        if (t + 1 > t_end) break;
-        step(t + 1);
-        if (t + 2 > t_end) break;
-        step(t + 2);
-        if (t + 3 > t_end) break;
-        step(t + 3);
-        if (t + 4 > t_end) break;
-        step(t + 4);
-        if (t + 5 > t_end) break;
-        step(t + 5);
+    // Wait once for the loads above: the loop's waits are then sized by the
+    // loads the loop itself keeps in flight (two and three rows ahead), not
+    // by the order in which the preloads happened to issue.  vmcnt(0), other
+    // counters left at their maxima (gfx9 encoding: expcnt 7, lgkmcnt 15).
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // Six row-steps per iteration: every ring has 2 or 3 slots, so each of
+    // the six phases addresses its slots at compile time.  The loop body has
+    // no exit (the remaining 0..5 row-steps follow it), so the waits on the
+    // loads in flight see one path around the loop.
+    int t = t_begin;
+    for (; t + 5 <= t_end; t += 6) {
+        step(std::integral_constant<int, 0>{}, t);
+        step(std::integral_constant<int, 1>{}, t + 1);
+        step(std::integral_constant<int, 2>{}, t + 2);
+        step(std::integral_constant<int, 3>{}, t + 3);
+        step(std::integral_constant<int, 4>{}, t + 4);
+        step(std::integral_constant<int, 5>{}, t + 5);
     }
+    if (t <= t_end) step(std::integral_constant<int, 0>{}, t);
+    if (t + 1 <= t_end) step(std::integral_constant<int, 1>{}, t + 1);
+    if (t + 2 <= t_end) step(std::integral_constant<int, 2>{}, t + 2);
+    if (t + 3 <= t_end) step(std::integral_constant<int, 3>{}, t + 3);
+    if (t + 4 <= t_end) step(std::integral_constant<int, 4>{}, t + 4);
     if (__ballot(bad) != 0 && lane == 0) atomicOr(g.status, 1u);
 }
 
@@ -383,6 +440,9 @@ bool stream_shape(int k, int* ox) {
     return stream_geom(k, ox, &w);
 }
 
+// Longest segment worth a single round of wavefronts (rows).
+static constexpr int kStreamSegRows = 320;
+
 StreamPlan stream_plan(int W, int H, int k, int cus) {
     StreamPlan p{};
     int ox = 0, per_simd = 0;
@@ -390,9 +450,17 @@ StreamPlan stream_plan(int W, int H, int k, int cus) {
     const int resident_waves = cus * 4 * per_simd;
     p.strips = (W + ox - 1) / ox;
     // one wavefront per segment, as many segments as fill the resident
-    // wavefront slots once, but no segment shorter than 4 k rows (its cone
-    // reaches 2 k rows into each neighbour)
-    int segs = std::max(1, resident_waves / p.strips);
+    // wavefront slots once -- or a whole number of times, so that no segment
+    // runs past kStreamSegRows rows -- but no segment shorter than 4 k rows
+    // (its cone reaches 2 k rows into each neighbour).  MI355X, closed sheet,
+    // k = 6, cell-updates/s by segment rows: 4096^2 81 rows (one round)
+    // 3.98e11, ahead of 41-162 rows; 8192^2 161 rows 4.61e11, 328 rows 4.55e11, 107
+    // rows 4.49e11; 16384^2 322 rows 5.24e11, 432 rows 5.17e11, 1366 rows
+    // 4.68e11.
+    const int segs1 = std::max(1, resident_waves / p.strips);
+    const int rows1 = (H + segs1 - 1) / segs1;
+    const int rounds = std::max(1, (rows1 + kStreamSegRows - 1) / kStreamSegRows);
+    int segs = segs1 * rounds;
     segs = std::min(segs, std::max(1, H / (4 * k)));
     p.seg_rows = (H + segs - 1) / segs;
     p.segs = (H + p.seg_rows - 1) / p.seg_rows;

@@ -6,7 +6,8 @@
 // 64 x 64 tiles (16 waves up to 512^2, 8 waves up to 2048^2: small grids
 // are bound by the latency of one tile's levels, so more and smaller tiles
 // finish sooner), 128 x 64 tiles beyond (two work groups per CU so one loads
-// while the other computes), 5 steps per launch up to 4096^2 and 4 above.
+// while the other computes), 5 steps per launch up to 4096^2 and 4 above;
+// grids above 3072^2 in closed walls take the streaming kernel at k = 6.
 // tws_autotune() times every candidate on scratch grids of the requested
 // shapes and returns the winner; only tws_tune_record() enters a launch into
 // the per-size-class table (log2 of the cell count) that choose_launch()
@@ -76,6 +77,21 @@ void choose_launch(int width, int height, int* kernel, int* k) {
             return;
         }
     }
+    // Above 3072^2 the streaming kernel at k = 6 leads (closed walls; other
+    // boundaries step on tile_launch's choice).  MI355X, 0.5 m sheet on fBm
+    // terrain in closed walls, cell-updates/s stream vs the tile table: 3072^2 3.36e11 vs 3.33e11,
+    // 4096^2 3.92e11 vs 3.67e11, 6144^2 4.53e11 vs 3.78e11, 8192^2 4.61e11 vs
+    // 3.98e11, 16384^2 5.24e11 vs 4.26e11; 2048^2 2.33e11 vs 2.90e11.
+    if (cells > 3072 * 3072) {
+        *kernel = TWS_KERNEL_STREAM_128;
+        *k = 6;
+        return;
+    }
+    tile_launch(width, height, kernel, k);
+}
+
+void tile_launch(int width, int height, int* kernel, int* k) {
+    const int64_t cells = (int64_t)width * height;
     // MI355X sweep (cell-updates/s; 0.5 m sheet on fBm terrain, with and without
     // a rain disc; stale halo rows skipped in the 64-wide tiles):
     //   <= 384^2   latency-bound, few tiles: 64x48 tiles over 16 waves of 3
@@ -128,6 +144,7 @@ void choose_launch(int width, int height, int* kernel, int* k) {
 // 2% of each other and 25 lost 3%).
 int default_tile_group(int width, int height, int kernel, int k) {
     const int64_t cells = (int64_t)width * height;
+    if (kernel == TWS_KERNEL_STREAM_128) tile_launch(width, height, &kernel, &k);
     if (kernel == TWS_KERNEL_TILE_128x64 && k == 5 && cells > 2048 * 2048 && cells <= 4096 * 4096)
         return 20;
     return 16;

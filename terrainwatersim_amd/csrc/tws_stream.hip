@@ -456,11 +456,13 @@ StreamPlan stream_plan(int W, int H, int k, int cus) {
     // k = 6, cell-updates/s by segment rows: 4096^2 81 rows (one round)
     // 3.98e11, ahead of 41-162 rows; 8192^2 161 rows 4.61e11, 328 rows 4.55e11, 107
     // rows 4.49e11; 16384^2 322 rows 5.24e11, 432 rows 5.17e11, 1366 rows
-    // 4.68e11.
+    // 4.68e11; 256 rows in 4.94 rounds 5.24e11 but 274 rows in 4.63 rounds
+    // 4.98e11.
     const int segs1 = std::max(1, resident_waves / p.strips);
     const int rows1 = (H + segs1 - 1) / segs1;
     const int rounds = std::max(1, (rows1 + kStreamSegRows - 1) / kStreamSegRows);
-    int segs = segs1 * rounds;
+    // whole rounds: the last round of a fractional count idles its slots
+    int segs = std::max(1, resident_waves * rounds / p.strips);
     segs = std::min(segs, std::max(1, H / (4 * k)));
     p.seg_rows = (H + segs - 1) / segs;
     p.segs = (H + p.seg_rows - 1) / p.seg_rows;

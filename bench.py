@@ -287,7 +287,9 @@ def main():
             rec["batched"] = workloads.batched(log=log)
         if "served" in extra:
             cap = rec.get("batched", {}).get("batched_cell_updates_per_s", value)
-            rec["served"] = workloads.served(cap, n_requests=args.served_requests, log=log)
+            dev_cap = max(value, rec.get("batched", {}).get("batched_cell_updates_per_s", 0.0))
+            rec["served"] = workloads.served(cap, n_requests=args.served_requests, log=log,
+                                             device_cap=dev_cap)
         if "large" in extra:
             rec["large"] = workloads.large(log=log)
     if not args.no_cpu_baseline:

@@ -345,9 +345,28 @@ int tws_batch_run(tws_batch* batch);
 /* tws_batch_set_steps(n) for every grid, then tws_batch_run. */
 int tws_batch_step(tws_batch* batch, int32_t n);
 /* Kernel variant and steps per launch for the whole batch (as tws_set_launch;
- * TWOPASS is not available for batches). */
+ * the tile kernels only: TWOPASS and STREAM_128 are not available for
+ * batches).  Same as tws_batch_set_launch_classes with one class.
+ * tws_batch_get_launch reports the class of the largest grids. */
 int tws_batch_set_launch(tws_batch* batch, int32_t kernel, int32_t steps_per_launch);
 int tws_batch_get_launch(const tws_batch* batch, int32_t* kernel, int32_t* steps_per_launch);
+
+#define TWS_BATCH_MAX_CLASSES 4
+/* Size classes, each with its own tile kernel and steps per launch: class c
+ * (0 <= c < n <= TWS_BATCH_MAX_CLASSES) steps the grids of at most
+ * max_cells[c] cells that no earlier class takes; the last class takes every
+ * larger grid (its max_cells entry is ignored).  Bounds must be positive and
+ * ascending.  Each class with pending steps gets its own chain of launches;
+ * the chains of a run are captured as parallel branches of one hipGraph.
+ * Results do not depend on the classes: every launch configuration steps a
+ * grid bit-identically.  Has no counterpart in the reference, which steps
+ * one terrain at a time. */
+int tws_batch_set_launch_classes(tws_batch* batch, int32_t n, const int64_t* max_cells,
+                                 const int32_t* kernels, const int32_t* steps_per_launch);
+/* The classes in effect: *n of them, the first min(*n, capacity) written to
+ * the arrays (any of which may be NULL). */
+int tws_batch_get_launch_classes(const tws_batch* batch, int32_t capacity, int64_t* max_cells,
+                                 int32_t* kernels, int32_t* steps_per_launch, int32_t* n);
 /* 1 (default): replay captured hipGraphs; 0: launch kernels directly. */
 int tws_batch_use_graphs(tws_batch* batch, int32_t enable);
 int tws_batch_stats_get(const tws_batch* batch, tws_batch_stats* out);

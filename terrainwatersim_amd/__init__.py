@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BOUNDARY_CLOSED, BOUNDARY_REFERENCE_DISPATCH, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x64,
+from ._lib import (BATCH_MAX_CLASSES, BOUNDARY_CLOSED, BOUNDARY_REFERENCE_DISPATCH, BOUNDARY_ZERO, KERNEL_AUTO, KERNEL_TILE_128x64,
                    KERNEL_TILE_128x128, KERNEL_TILE_256x64, KERNEL_TILE_64x64,
                    KERNEL_TILE_64x64_16W, KERNEL_TILE_64x48_16W, KERNEL_STREAM_128, KERNEL_TWOPASS, LIB_PATH,
                    PROTOTYPES, REQ_CANCELLED, REQ_DONE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING,
@@ -24,7 +24,7 @@ __all__ = [
     "BOUNDARY_CLOSED", "BOUNDARY_ZERO", "BOUNDARY_REFERENCE_DISPATCH", "KERNEL_AUTO", "KERNEL_TWOPASS", "KERNEL_TILE_128x128",
     "KERNEL_TILE_128x64", "KERNEL_TILE_256x64", "KERNEL_TILE_64x64", "KERNEL_TILE_64x64_16W",
     "KERNEL_TILE_64x48_16W", "KERNEL_STREAM_128",
-    "TILE_KERNELS",
+    "TILE_KERNELS", "BATCH_MAX_CLASSES",
 ]
 
 
@@ -423,6 +423,25 @@ class Batch:
         k, s = C.c_int32(), C.c_int32()
         check(lib().tws_batch_get_launch(self._h, C.byref(k), C.byref(s)))
         return k.value, s.value
+
+    def set_launch_classes(self, classes):
+        """Size classes [(max_cells, kernel, steps_per_launch), ...], ascending
+        by max_cells; the last class takes every larger grid."""
+        n = len(classes)
+        mc = np.array([int(c[0]) for c in classes], np.int64)
+        kern = np.array([int(c[1]) for c in classes], np.int32)
+        ks = np.array([int(c[2]) for c in classes], np.int32)
+        check(lib().tws_batch_set_launch_classes(self._h, n, mc.ctypes.data, kern.ctypes.data,
+                                                 ks.ctypes.data))
+
+    def get_launch_classes(self):
+        n = C.c_int32()
+        mc = np.zeros(BATCH_MAX_CLASSES, np.int64)
+        kern = np.zeros(BATCH_MAX_CLASSES, np.int32)
+        ks = np.zeros(BATCH_MAX_CLASSES, np.int32)
+        check(lib().tws_batch_get_launch_classes(self._h, BATCH_MAX_CLASSES, mc.ctypes.data,
+                                                 kern.ctypes.data, ks.ctypes.data, C.byref(n)))
+        return [(int(mc[c]), int(kern[c]), int(ks[c])) for c in range(n.value)]
 
     def use_graphs(self, enable: bool):
         check(lib().tws_batch_use_graphs(self._h, 1 if enable else 0))

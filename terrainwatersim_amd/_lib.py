@@ -34,6 +34,7 @@ KERNEL_TILE_64x64 = 5
 KERNEL_TILE_64x64_16W = 6
 KERNEL_TILE_64x48_16W = 7
 KERNEL_STREAM_128 = 8
+BATCH_MAX_CLASSES = 4
 TILE_KERNELS = (KERNEL_TILE_128x128, KERNEL_TILE_128x64, KERNEL_TILE_256x64, KERNEL_TILE_64x64,
                 KERNEL_TILE_64x64_16W, KERNEL_TILE_64x48_16W)
 
@@ -184,6 +185,8 @@ PROTOTYPES = {
     "tws_batch_step": (C.c_int, [_vp, _i32]),
     "tws_batch_set_launch": (C.c_int, [_vp, _i32, _i32]),
     "tws_batch_get_launch": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32)]),
+    "tws_batch_set_launch_classes": (C.c_int, [_vp, _i32, _vp, _vp, _vp]),
+    "tws_batch_get_launch_classes": (C.c_int, [_vp, _i32, _vp, _vp, _vp, C.POINTER(_i32)]),
     "tws_batch_use_graphs": (C.c_int, [_vp, _i32]),
     "tws_batch_stats_get": (C.c_int, [_vp, C.POINTER(BatchStats)]),
     "tws_batch_sync": (C.c_int, [_vp]),

@@ -1,17 +1,21 @@
 // tws_batch.hip -- many independent grids advanced by one launch sequence.
 //
-// Each grid is a GridState carved from the batch's device pool.  A run
-// flattens the tiles of every grid with pending steps into one work list
-// (GridDesc table + {grids, tiles} header in device memory, ordered by
-// tile_begin) and launches ceil(max_steps / k) tile kernels; launch j moves
-// grid g by min(k, steps_g - j*k) steps, so grids with fewer steps simply
-// drop out of the later launches.  The launch sequence reads its table at
-// run time, so the captured hipGraph of a (kernel, k, launches, block bound)
-// shape is replayed for any table that fits it.
+// Each grid is a GridState carved from the batch's device pool.  Grids fall
+// into size classes (by cell count), each with its own tile kernel and steps
+// per launch.  A run flattens the tiles of each class's grids with pending
+// steps into one work list (a segment of the GridDesc table plus a {grids,
+// tiles, offset} header in device memory, ordered by tile_begin) and launches
+// ceil(max_steps / k) tile kernels for it; launch j moves grid g by
+// min(k, steps_g - j*k) steps, so grids with fewer steps simply drop out of
+// the later launches.  The classes' launch chains are independent (disjoint
+// grids) and are captured as parallel branches of one hipGraph.  The chains
+// read their tables at run time, so a captured graph of a shape (per class:
+// kernel, k, launches, block bound, mode) is replayed for any table that fits it.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <map>
 #include <new>
 #include <tuple>
@@ -33,13 +37,23 @@ struct Slot {
     int steps = 0;  // pending for the next run
 };
 
-struct GraphKey {
-    int kernel, k, launches, blocks, mode;
-    bool operator<(const GraphKey& o) const {
-        return std::tie(kernel, k, launches, blocks, mode) <
-               std::tie(o.kernel, o.k, o.launches, o.blocks, o.mode);
+// A size class: grids of at most max_cells cells not taken by an earlier class.
+struct LaunchClass {
+    int64_t max_cells;
+    int kernel, k;
+};
+
+// One class's launch chain in a run.
+struct Chain {
+    int cls, kernel, k, launches, blocks, mode;
+    bool operator<(const Chain& o) const {
+        return std::tie(cls, kernel, k, launches, blocks, mode) <
+               std::tie(o.cls, o.kernel, o.k, o.launches, o.blocks, o.mode);
     }
 };
+using GraphKey = std::vector<Chain>;
+
+constexpr int kHdrInts = 4;  // per class: grids, tiles, table offset, unused
 
 }  // namespace
 
@@ -56,8 +70,10 @@ struct tws_batch {
     GridDesc* table_host = nullptr;  // pinned
     int* hdr_host = nullptr;         // pinned
     hipEvent_t table_copied = nullptr;
-    int kernel = TWS_KERNEL_TILE_128x64;
-    int k = 4;
+    std::vector<LaunchClass> classes{{INT64_MAX, TWS_KERNEL_TILE_128x64, 4}};
+    hipStream_t aux[TWS_BATCH_MAX_CLASSES - 1] = {};  // branch streams of the chains
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[TWS_BATCH_MAX_CLASSES - 1] = {};
     bool use_graphs = true;
     std::map<GraphKey, hipGraphExec_t> graphs;
     int64_t captures = 0, replays = 0;
@@ -90,25 +106,42 @@ static void drop_graphs(tws_batch* b) {
     b->graphs.clear();
 }
 
+// Issues the chains of one run: the first on the batch stream, the others on
+// branch streams forked from it and joined back, so they may run side by side.
+static hipError_t issue_chains(tws_batch* b, const GraphKey& chains) {
+    hipError_t e = hipSuccess;
+    const int n = (int)chains.size();
+    if (n > 1) {
+        e = hipEventRecord(b->fork, b->stream);
+        for (int c = 1; c < n && e == hipSuccess; ++c) e = hipStreamWaitEvent(b->aux[c - 1], b->fork, 0);
+    }
+    for (int c = 0; c < n && e == hipSuccess; ++c) {
+        const Chain& ch = chains[c];
+        hipStream_t st = c == 0 ? b->stream : b->aux[c - 1];
+        for (int j = 0; j < ch.launches && e == hipSuccess; ++j)
+            e = launch_tile_batch(ch.kernel, ch.k, ch.mode, b->table_dev, b->hdr_dev + kHdrInts * ch.cls,
+                                  ch.blocks, j, st);
+    }
+    for (int c = 1; c < n && e == hipSuccess; ++c) {
+        e = hipEventRecord(b->join[c - 1], b->aux[c - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join[c - 1], 0);
+    }
+    return e;
+}
+
 // Launches (or replays) the tile kernels of one run.
-static int run_launches(tws_batch* b, int launches, int blocks, int mode) {
+static int run_launches(tws_batch* b, const GraphKey& chains) {
     if (!b->use_graphs) {
-        for (int j = 0; j < launches; ++j)
-            TWS_HIP(launch_tile_batch(b->kernel, b->k, mode, b->table_dev, b->hdr_dev, blocks,
-                                      j, b->stream), "batch: tile kernel");
+        TWS_HIP(issue_chains(b, chains), "batch: tile kernels");
         return TWS_OK;
     }
-    const GraphKey key{b->kernel, b->k, launches, blocks, mode};
-    auto it = b->graphs.find(key);
+    auto it = b->graphs.find(chains);
     if (it == b->graphs.end()) {
         if (b->graphs.size() >= 32) drop_graphs(b);
         hipGraph_t graph = nullptr;
         TWS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
                 "batch: begin capture");
-        hipError_t e = hipSuccess;
-        for (int j = 0; j < launches && e == hipSuccess; ++j)
-            e = launch_tile_batch(b->kernel, b->k, mode, b->table_dev, b->hdr_dev, blocks, j,
-                                  b->stream);
+        hipError_t e = issue_chains(b, chains);
         hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
         if (e != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return hip_fail(e, "batch: capture"); }
         if (e2 != hipSuccess) return hip_fail(e2, "batch: end capture");
@@ -117,7 +150,7 @@ static int run_launches(tws_batch* b, int launches, int blocks, int mode) {
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return hip_fail(e, "batch: instantiate graph");
         try {
-            it = b->graphs.emplace(key, exec).first;
+            it = b->graphs.emplace(chains, exec).first;
         } catch (...) {
             (void)hipGraphExecDestroy(exec);
             return fail(TWS_ERR_NOMEM, "batch: graph cache");
@@ -127,6 +160,14 @@ static int run_launches(tws_batch* b, int launches, int blocks, int mode) {
     TWS_HIP(hipGraphLaunch(it->second, b->stream), "batch: graph launch");
     ++b->replays;
     return TWS_OK;
+}
+
+// Index of the class that steps a grid of `cells` cells.
+static int class_of(const tws_batch* b, int64_t cells) {
+    const int n = (int)b->classes.size();
+    for (int c = 0; c + 1 < n; ++c)
+        if (cells <= b->classes[c].max_cells) return c;
+    return n - 1;
 }
 
 extern "C" {
@@ -190,6 +231,11 @@ int tws_batch_destroy(tws_batch* b) {
     if (b->table_host) (void)hipHostFree(b->table_host);
     if (b->hdr_host) (void)hipHostFree(b->hdr_host);
     if (b->table_copied) (void)hipEventDestroy(b->table_copied);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (int c = 0; c < TWS_BATCH_MAX_CLASSES - 1; ++c) {
+        if (b->join[c]) (void)hipEventDestroy(b->join[c]);
+        if (b->aux[c]) (void)hipStreamDestroy(b->aux[c]);
+    }
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return TWS_OK;
@@ -320,38 +366,48 @@ int tws_batch_run(tws_batch* b) {
     if (!b) return fail(TWS_ERR_INVALID, "tws_batch_run: NULL handle");
     // the pinned table may still be in flight from the previous run
     TWS_HIP(hipEventSynchronize(b->table_copied), "tws_batch_run: wait table");
-    int n = 0, tiles = 0, max_steps = 0;
-    int mode = kModePlain;
-    for (int i = 0; i < b->max_grids; ++i) {
-        Slot& s = b->slots[i];
-        if (!s.used || s.steps <= 0) continue;
-        GridDesc g = s.g.desc();
-        if (!grid_tiles(s.g.W, s.g.H, b->kernel, b->k, &g.tiles_x, &g.tiles_y))
-            return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
-        g.tile_begin = tiles;
-        g.steps = s.steps;
-        mode = std::max(mode, kernel_mode(g));
-        tiles += g.tiles_x * g.tiles_y;
-        max_steps = std::max(max_steps, s.steps);
-        b->table_host[n++] = g;
+    const int nc = (int)b->classes.size();
+    int n = 0;
+    GraphKey chains;
+    for (int c = 0; c < nc; ++c) {
+        const LaunchClass& lc = b->classes[c];
+        const int first = n;
+        int tiles = 0, max_steps = 0, mode = kModePlain;
+        for (int i = 0; i < b->max_grids; ++i) {
+            Slot& s = b->slots[i];
+            if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c) continue;
+            GridDesc g = s.g.desc();
+            if (!grid_tiles(s.g.W, s.g.H, lc.kernel, lc.k, &g.tiles_x, &g.tiles_y))
+                return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
+            g.tile_begin = tiles;
+            g.steps = s.steps;
+            mode = std::max(mode, kernel_mode(g));
+            tiles += g.tiles_x * g.tiles_y;
+            max_steps = std::max(max_steps, s.steps);
+            b->table_host[n++] = g;
+        }
+        b->hdr_host[kHdrInts * c + 0] = n - first;
+        b->hdr_host[kHdrInts * c + 1] = tiles;
+        b->hdr_host[kHdrInts * c + 2] = first;
+        b->hdr_host[kHdrInts * c + 3] = 0;
+        if (n == first) continue;
+        // block bound rounded up so small changes of the tile count reuse a graph
+        chains.push_back(Chain{c, lc.kernel, lc.k, (max_steps + lc.k - 1) / lc.k,
+                               (tiles + 511) / 512 * 512, mode});
     }
     if (n == 0) return TWS_OK;
-    b->hdr_host[0] = n;
-    b->hdr_host[1] = tiles;
     TWS_HIP(hipMemcpyAsync(b->table_dev, b->table_host, sizeof(GridDesc) * (size_t)n,
                            hipMemcpyHostToDevice, b->stream), "tws_batch_run: table");
-    TWS_HIP(hipMemcpyAsync(b->hdr_dev, b->hdr_host, 2 * sizeof(int), hipMemcpyHostToDevice,
-                           b->stream), "tws_batch_run: header");
+    TWS_HIP(hipMemcpyAsync(b->hdr_dev, b->hdr_host, sizeof(int) * kHdrInts * (size_t)nc,
+                           hipMemcpyHostToDevice, b->stream), "tws_batch_run: header");
     TWS_HIP(hipEventRecord(b->table_copied, b->stream), "tws_batch_run: event");
-    const int launches = (max_steps + b->k - 1) / b->k;
-    // block bound rounded up so small changes of the tile count reuse a graph
-    const int blocks = (tiles + 511) / 512 * 512;
-    int rc = run_launches(b, launches, blocks, mode);
+    int rc = run_launches(b, chains);
     if (rc != TWS_OK) return rc;
     for (int i = 0; i < b->max_grids; ++i) {
         Slot& s = b->slots[i];
         if (!s.used || s.steps <= 0) continue;
-        const int gl = (s.steps + b->k - 1) / b->k;
+        const int k = b->classes[class_of(b, (int64_t)s.g.W * s.g.H)].k;
+        const int gl = (s.steps + k - 1) / k;
         s.g.parity ^= (gl & 1);
         s.steps = 0;
     }
@@ -371,19 +427,64 @@ int tws_batch_set_launch(tws_batch* b, int32_t kernel, int32_t k) {
         kernel = TWS_KERNEL_TILE_128x64;
         k = 4;
     }
-    TileShape sh{};
-    if (!tile_shape(kernel, k, &sh))
-        return fail(TWS_ERR_INVALID, "tws_batch_set_launch: kernel %d / k %d not available for "
-                    "batches", kernel, k);
-    b->kernel = kernel;
-    b->k = k;
-    return TWS_OK;
+    const int64_t max_cells = INT64_MAX;
+    return tws_batch_set_launch_classes(b, 1, &max_cells, &kernel, &k);
 }
 
 int tws_batch_get_launch(const tws_batch* b, int32_t* kernel, int32_t* k) {
     if (!b) return fail(TWS_ERR_INVALID, "tws_batch_get_launch: NULL handle");
-    if (kernel) *kernel = b->kernel;
-    if (k) *k = b->k;
+    if (kernel) *kernel = b->classes.back().kernel;
+    if (k) *k = b->classes.back().k;
+    return TWS_OK;
+}
+
+int tws_batch_set_launch_classes(tws_batch* b, int32_t n, const int64_t* max_cells,
+                                 const int32_t* kernels, const int32_t* ks) {
+    if (!b || !max_cells || !kernels || !ks)
+        return fail(TWS_ERR_INVALID, "tws_batch_set_launch_classes: NULL argument");
+    if (n < 1 || n > TWS_BATCH_MAX_CLASSES)
+        return fail(TWS_ERR_INVALID, "tws_batch_set_launch_classes: %d classes (1..%d)", n,
+                    TWS_BATCH_MAX_CLASSES);
+    for (int c = 0; c < n; ++c) {
+        TileShape sh{};
+        if (!tile_shape(kernels[c], ks[c], &sh))
+            return fail(TWS_ERR_INVALID, "tws_batch_set_launch_classes: class %d: kernel %d / k %d "
+                        "not available for batches", c, kernels[c], ks[c]);
+        if (c + 1 < n && (max_cells[c] <= 0 || (c > 0 && max_cells[c] <= max_cells[c - 1])))
+            return fail(TWS_ERR_INVALID, "tws_batch_set_launch_classes: class bounds must be "
+                        "positive and ascending");
+    }
+    // branch streams and events for the chains beyond the first, made once
+    for (int c = 0; c + 1 < n; ++c) {
+        if (!b->aux[c]) TWS_HIP(hipStreamCreateWithFlags(&b->aux[c], hipStreamNonBlocking),
+                                "tws_batch_set_launch_classes: stream");
+        if (!b->join[c]) TWS_HIP(hipEventCreateWithFlags(&b->join[c], hipEventDisableTiming),
+                                 "tws_batch_set_launch_classes: event");
+    }
+    if (n > 1 && !b->fork)
+        TWS_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming),
+                "tws_batch_set_launch_classes: event");
+    std::vector<LaunchClass> cl;
+    try {
+        for (int c = 0; c < n; ++c)
+            cl.push_back(LaunchClass{c + 1 < n ? max_cells[c] : INT64_MAX, kernels[c], ks[c]});
+    } catch (...) {
+        return fail(TWS_ERR_NOMEM, "tws_batch_set_launch_classes: host allocation failed");
+    }
+    b->classes.swap(cl);
+    return TWS_OK;
+}
+
+int tws_batch_get_launch_classes(const tws_batch* b, int32_t capacity, int64_t* max_cells,
+                                 int32_t* kernels, int32_t* ks, int32_t* n) {
+    if (!b || !n) return fail(TWS_ERR_INVALID, "tws_batch_get_launch_classes: NULL argument");
+    const int nc = (int)b->classes.size();
+    *n = nc;
+    for (int c = 0; c < nc && c < capacity; ++c) {
+        if (max_cells) max_cells[c] = b->classes[c].max_cells;
+        if (kernels) kernels[c] = b->classes[c].kernel;
+        if (ks) ks[c] = b->classes[c].k;
+    }
     return TWS_OK;
 }
 

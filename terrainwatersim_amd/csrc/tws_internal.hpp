@@ -95,7 +95,8 @@ bool tile_shape(int kernel, int k, TileShape* out);
 constexpr int kModePlain = 0, kModeRain = 1, kModeGeneral = 2;
 
 // Launchers (stream-ordered, return hipError_t).
-// Batch launch: hdr_dev = {number of grids, number of tiles}; `blocks` is an
+// Batch launch: hdr_dev = {number of grids, number of tiles, offset of the
+// first grid in grids_dev}, the grids ordered by tile_begin; `blocks` is an
 // upper bound of the tile count (extra blocks exit).  `mode` must be at least
 // kernel_mode(g) of every grid of the table.
 int kernel_mode(const GridDesc& g);

@@ -315,8 +315,8 @@ __device__ __forceinline__ bool tile_step(const GridDesc& g, int tx, int ty, int
 // Fused, temporally blocked step.  Grid: one block per tile of the flattened
 // tile list.  Single grid: `grids` is null, g1 (by value) is the grid and
 // the tile count is gridDim.x.  Batch: hdr = {number of grids, number of
-// tiles} and grids[] (ordered by tile_begin) live in device memory, so a
-// captured launch stays valid while grids come and go; blocks past the tile
+// tiles, offset} and grids[offset..] (ordered by tile_begin) live in device
+// memory, so a captured launch stays valid while grids come and go; blocks past the tile
 // count exit at once.  MODE is the highest kernel_mode of the launch's grids
 // (chosen on the host from the descriptors).
 // At least 4 waves per SIMD (at most 128 VGPRs): 2 work groups of 512 per CU.
@@ -329,6 +329,7 @@ k_tile(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ 
     // halos overlap) share one L2.
     const int ngrids = grids ? hdr[0] : 0;
     const int total_tiles = grids ? hdr[1] : (int)gridDim.x;
+    if (grids) grids += hdr[2];  // this launch's segment of the batch table
     const int bid = blockIdx.x;
     if (bid >= total_tiles) return;
     const int xcd = bid & 7, idx = bid >> 3;

@@ -5,8 +5,11 @@ small    256^2 fBm terrain (seed 1), dry start, rain disc r = 16 (R^2 = 256)
          1000 steps: the latency/launch-bound case (state fits in L2/MALL).
 batched  256 independent terrains, sides log-uniform 128..2048 (seed 3),
          per-grid random rain discs (r in [8, 32], 1e-3 m/step), 100 steps per
-         captured launch sequence; (variant, k) from the autotuner.  Compared
-         with the same grids stepped one at a time.
+         captured launch sequence.  Three size classes (sides <= 512, <= 1024,
+         larger), each on the (variant, k) the autotuner picks for its grids,
+         as concurrent launch chains; beside it the whole mix on the one
+         launch the autotuner picks for all of it.  Compared with the same
+         grids stepped one at a time.
 served   Poisson stream of solves, sides log-uniform 256..4096, steps uniform
          100..5000, dam-break (20 m column over a centred square of side/8)
          or rain (r = 16 at the centre, 1e-3 m/step) 50/50, arrival rate 70%
@@ -100,6 +103,24 @@ def batch_grids(n_grids=256, seed=3, only=None):
     return grids
 
 
+# Size classes of the batched workload, by grid side: <= 512, <= 1024, larger.
+BATCH_CLASS_SIDES = (512, 1024)
+
+
+def _time_runs(batch, steps, repeats):
+    """Median seconds of `repeats` runs of `steps` steps after one warm run
+    (which captures the launch graph)."""
+    batch.step(steps)
+    batch.sync()
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        batch.step(steps)
+        batch.sync()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
 def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
     p = tws.params_physical(**PARAMS)
     grids = batch_grids(n_grids, seed)
@@ -108,26 +129,35 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
     # tuned on the step count of the runs it serves: a trial whose steps leave
     # a partial last launch would misjudge k
     best, trials = tws.autotune([(s, s) for s, _, _ in grids], steps=steps)
+    classes, class_recs, lo = [], [], 0
+    for hi in [c * c for c in BATCH_CLASS_SIDES] + [None]:
+        shapes = [(s, s) for s, _, _ in grids if s * s > lo and (hi is None or s * s <= hi)]
+        if shapes:
+            cb, _ = tws.autotune(shapes, steps=steps)
+            classes.append((hi or 0, cb["kernel"], cb["k"]))
+            class_recs.append({"max_cells": hi, "grids": len(shapes),
+                               "cells": int(sum(w * h for w, h in shapes)), "autotune_best": cb})
+        lo = hi or lo
+    if classes:
+        classes[-1] = (0,) + classes[-1][1:]
     pool = sum(tws.grid_footprint(s, s) for s, _, _ in grids)
     out = {"grids": n_grids, "cells": cells, "steps_per_run": steps,
-           "autotune_best": best, "autotune_trials": trials}
+           "autotune_best": best, "autotune_trials": trials, "classes": class_recs}
     with tws.Batch(n_grids, pool) as batch:
-        ids = []
-        for s, b, rain in grids:
-            gid = batch.add(s, s, p)
-            batch.upload(gid, b, np.zeros((s, s), F32))
-            batch.set_rain(gid, *rain)
-            ids.append(gid)
+        ids = [batch.add(s, s, p) for s, _, _ in grids]
+
+        def load():
+            for gid, (s, b, rain) in zip(ids, grids):
+                batch.upload(gid, b, np.zeros((s, s), F32))
+                batch.set_rain(gid, *rain)
+
+        load()
         batch.set_launch(best["kernel"], best["k"])
-        batch.step(steps)  # capture + warm
-        batch.sync()
-        times = []
-        for _ in range(repeats):
-            t0 = time.perf_counter()
-            batch.step(steps)
-            batch.sync()
-            times.append(time.perf_counter() - t0)
-        t = float(np.median(times))
+        t1 = _time_runs(batch, steps, repeats)
+        out["batched_one_launch_cell_updates_per_s"] = cells * steps / t1
+        load()
+        batch.set_launch_classes(classes)
+        t = _time_runs(batch, steps, repeats)
         out["batched_cell_updates_per_s"] = cells * steps / t
         out["batched_ms_per_run"] = t * 1e3
         out["graph"] = batch.stats()
@@ -147,6 +177,8 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
         if not forced:
             out["plain_launches"] = launches
     out["batched_over_plain"] = out["batched_cell_updates_per_s"] / out["plain_cell_updates_per_s"]
+    out["classes_over_one_launch"] = (out["batched_cell_updates_per_s"]
+                                      / out["batched_one_launch_cell_updates_per_s"])
     out["batched_over_plain_forced"] = (out["batched_cell_updates_per_s"]
                                         / out["plain_forced_cell_updates_per_s"])
     out["batched_equals_plain"] = bool(out["plain_equals_batched"]
@@ -227,7 +259,8 @@ def _pcts(lat):
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
 
 
-def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log=print):
+def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log=print,
+           device_cap=None):
     p = tws.params_physical(**PARAMS)
     probe = Requests(n_requests, seed, 1.0)
     mean_work = probe.work() / n_requests
@@ -307,6 +340,14 @@ def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log
     res["served_capacity_over_plain"] = (res["served_saturated"]["cell_updates_per_s"]
                                          / res["plain"]["cell_updates_per_s"])
     res["served_equals_plain"] = bool(same)
+    if device_cap:
+        # No schedule steps the stream faster than the device's best rate on
+        # its sizes, so served/plain can reach at most cap / plain rate.
+        res["served_over_plain_bound"] = {
+            "device_cap_cell_updates_per_s": device_cap,
+            "max_ratio": device_cap / res["plain"]["cell_updates_per_s"],
+            "cap_source": "the fastest measured rate on the stream's sizes (the 4096^2 "
+                          "single-grid step, or the batched mix if faster)"}
     return res
 
 

@@ -227,3 +227,64 @@ def test_batch_refuses_truncating_dispatch_grids(gpu):
         with pytest.raises(gpu.TwsError):
             bt.set_params(other, p)
         assert gid != other
+
+
+CLASS_SHAPES = [(1, 1), (37, 53), (96, 100), (256, 256), (129, 300), (300, 520), (700, 1024),
+                (64, 2048)]
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_launch_classes_bitwise(gpu, oracle, graphs):
+    """A batch whose grids fall into four size classes, each on its own tile
+    kernel and steps per launch (concurrent launch chains), with per-grid step
+    counts: three runs in a row, the eager downloads between them, and every
+    grid after every run bit-identical to the FP32 restatement."""
+    KT = gpu
+    classes = [(100 * 100, KT.KERNEL_TILE_64x48_16W, 8), (300 * 300, KT.KERNEL_TILE_64x64_16W, 3),
+               (600 * 600, KT.KERNEL_TILE_64x64, 5), (0, KT.KERNEL_TILE_128x64, 4)]
+    grids = []
+    for i, (H, W) in enumerate(CLASS_SHAPES):
+        b, d, f = state(300 + i, H, W)
+        boundary = gpu.BOUNDARY_ZERO if i % 3 == 1 else gpu.BOUNDARY_CLOSED
+        p = gpu.params_physical(0.02 + 0.003 * i, 9.81, 1.0, 1.0, 0.995, boundary)
+        rain = (W / 2.0, H / 3.0, 1e-2, 64.0) if i % 2 == 0 else None
+        grids.append((b, d, f, p, rain))
+    steps = [7, 13, 5, 11, 9, 3, 6, 10]
+    oracle.set_threads(16)
+    want = [(d.copy(), f.copy()) for _, d, f, _, _ in grids]
+    with gpu.Batch(16, 1 * GB) as batch:
+        batch.use_graphs(graphs)
+        batch.set_launch_classes(classes)
+        got = batch.get_launch_classes()
+        assert [c[1:] for c in got] == [c[1:] for c in classes]
+        assert [c[0] for c in got[:3]] == [c[0] for c in classes[:3]] and got[3][0] == 2**63 - 1
+        assert batch.get_launch() == (KT.KERNEL_TILE_128x64, 4)
+        ids = load_batch(gpu, batch, grids)
+        for run in range(3):
+            for gid, n in zip(ids, steps):
+                batch.set_steps(gid, n)
+            batch.run()
+            for j, ((b, _, _, p, rain), gid, n) in enumerate(zip(grids, ids, steps)):
+                F, K, Cc, bd, _ = p.as_tuple()
+                d32, f32, _ = oracle.step(b, want[j][0], want[j][1], F, K, Cc, n=n, boundary=bd,
+                                          rain=rain)
+                want[j] = (d32, f32)
+                out = batch.download(gid, flux=True)
+                assert np.array_equal(out["depth"], d32), (run, CLASS_SHAPES[j])
+                assert np.array_equal(out["flux"], f32), (run, CLASS_SHAPES[j])
+        st = batch.stats()
+        if graphs:
+            assert st["graph_captures"] == 1 and st["graph_replays"] == 3
+
+
+def test_launch_classes_rejects(gpu):
+    with gpu.Batch(4, 64 << 20) as batch:
+        T = gpu.KERNEL_TILE_128x64
+        for bad in ([], [(0, T, 4)] * 5, [(500, T, 4), (400, T, 4), (0, T, 4)],
+                    [(0, T, 4), (0, T, 4)], [(0, gpu.KERNEL_STREAM_128, 6)], [(0, T, 7)],
+                    [(0, gpu.KERNEL_TWOPASS, 1)]):
+            with pytest.raises(gpu.TwsError):
+                batch.set_launch_classes(bad)
+        assert batch.get_launch() == (T, 4)
+        batch.set_launch(gpu.KERNEL_TILE_64x64, 5)
+        assert batch.get_launch_classes() == [(2**63 - 1, gpu.KERNEL_TILE_64x64, 5)]

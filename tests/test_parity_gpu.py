@@ -230,6 +230,61 @@ def test_full_size_4096_window_vs_restatements(gpu, oracle):
     assert err < 1e-4 * m
 
 
+@pytest.mark.timeout(1200)
+def test_config2_full_domain_vs_restatements(gpu, oracle, capsys):
+    """The 4096^2 sheet workload (fBm seed 2 + 0.5 m, closed walls) for its
+    full 1000 steps on the default launch: the whole depth and flux state is
+    bit-identical to the FP32 restatement, and its FP32 error against the FP64
+    restatement over the whole domain stays within the stated bounds.
+
+    FP32 vs FP64 here is the model's own sensitivity to rounding, which the
+    shaders share (they compute the same FP32 values): a thin sheet draining
+    over steep terrain is dominated by wet/dry fronts, where the flux scaling
+    (flowUpdate.comp:58-59) and the max(0, .) clamps switch, so a rounding
+    difference moves a front by a cell.  Measured (host FP64 restatement,
+    1000 steps): median abs 1.5e-6 m, p99 0.011 m, p99.9 0.055 m, max 0.80 m
+    (87 of 16.8 M cells above 0.5 m; max depth 5.49 m); volume equal to
+    1.6e-8 relative."""
+    N, steps, chunk = 4096, 1000, 100
+    b = gpu.generate_fbm_terrain(N, N, seed=2)
+    d = np.full((N, N), 0.5, F32)
+    p = gpu.params_physical(**CFG_PARAMS)
+    with gpu.Simulation(N, N, p) as s:
+        s.upload(b, d)
+        s.step(steps)
+        out = s.download(flux=True)
+        assert s.check_finite()
+    oracle.set_threads(16)
+    F, K, Cc = p3(p)
+
+    def restate(dtype):
+        dd, ff = d, None
+        for i in range(0, steps, chunk):
+            dd, ff, _ = oracle.step(b, dd, ff, F, K, Cc, n=chunk, dtype=dtype)
+            with capsys.disabled():
+                print(f"  config 2 {np.dtype(dtype).name} restatement: {i + chunk}/{steps} steps",
+                      flush=True)
+        return dd, ff
+
+    d32, f32 = restate(np.float32)
+    assert np.array_equal(out["depth"], d32)
+    assert np.array_equal(out["flux"], f32)
+    del d32, f32
+    d64, _ = restate(np.float64)
+    err = np.abs(out["depth"].astype(np.float64) - d64)
+    q50, q99, q999 = (float(np.quantile(err, q)) for q in (0.5, 0.99, 0.999))
+    v0 = float(d.sum(dtype=np.float64))
+    v32 = float(out["depth"].sum(dtype=np.float64))
+    v64 = float(d64.sum())
+    with capsys.disabled():
+        print(f"  config 2, 1000 steps, FP32 vs FP64 over 4096^2: median {q50:.2e} m, p99 "
+              f"{q99:.2e} m, p99.9 {q999:.2e} m, max {err.max():.3f} m (max depth "
+              f"{d64.max():.2f} m); volume FP32 {abs(v32 - v0) / v0:.1e}, FP64 "
+              f"{abs(v64 - v0) / v0:.1e} relative drift", flush=True)
+    assert q50 < 1e-5 and q99 < 0.05 and q999 < 0.1 and err.max() < 1.5
+    assert abs(v32 - v0) / v0 < 1e-6 and abs(v64 - v0) / v0 < 1e-9
+
+
 def test_split_run_equals_continuous(gpu):
     b, d, f = random_state(5, 300, 200)
     p = gpu.params_physical(**CFG_PARAMS)

@@ -101,15 +101,17 @@ __device__ __forceinline__ bool brush_reaches(float cx, float cy, float R2, int 
     return dx * dx + dy * dy < (double)R2 * (1.0 + 1e-6);
 }
 
-// flowUpdate.comp:53-59 for one cell: new outgoing flux from the four head
-// differences, clamped at zero and scaled so the cell cannot drain below 0.
-__device__ __forceinline__ void flux_update(float F, float K, float C, float d, float g0,
-                                            float g1, float g2, float g3, float& f0, float& f1,
-                                            float& f2, float& f3) {
-    float o0 = fmaxf(0.0f, f0 * F + g0 * K);
-    float o1 = fmaxf(0.0f, f1 * F + g1 * K);
-    float o2 = fmaxf(0.0f, f2 * F + g2 * K);
-    float o3 = fmaxf(0.0f, f3 * F + g3 * K);
+// flowUpdate.comp:53-59 for one cell: new outgoing flux from the products
+// p_i = g_i * K of the four head differences g_i (own head minus the
+// neighbour's, +x, -x, +y, -y), clamped at zero and scaled so the cell cannot
+// drain below 0.
+__device__ __forceinline__ void flux_update_p(float F, float C, float d, float p0, float p1,
+                                              float p2, float p3, float& f0, float& f1,
+                                              float& f2, float& f3) {
+    float o0 = fmaxf(0.0f, f0 * F + p0);
+    float o1 = fmaxf(0.0f, f1 * F + p1);
+    float o2 = fmaxf(0.0f, f2 * F + p2);
+    float o3 = fmaxf(0.0f, f3 * F + p3);
     // flowUpdate.comp:57 summed pairwise, the order Mesa executes the reference in
     const float total = ((o0 + o1) + (o2 + o3)) * C;
     if (total > d) {
@@ -120,6 +122,13 @@ __device__ __forceinline__ void flux_update(float F, float K, float C, float d, 
         o0 = o0 * s; o1 = o1 * s; o2 = o2 * s; o3 = o3 * s;
     }
     f0 = o0; f1 = o1; f2 = o2; f3 = o3;
+}
+
+// The same from the head differences g_i themselves.
+__device__ __forceinline__ void flux_update(float F, float K, float C, float d, float g0,
+                                            float g1, float g2, float g3, float& f0, float& f1,
+                                            float& f2, float& f3) {
+    flux_update_p(F, C, d, g0 * K, g1 * K, g2 * K, g3 * K, f0, f1, f2, f3);
 }
 
 }  // namespace tws

@@ -40,10 +40,10 @@ constexpr int kStreamWaves = 4;          // independent wavefronts per work grou
 
 template <int KH>
 struct StreamGeom {
-    // Wavefronts per SIMD: a level's window takes about 32 VGPRs, so up to 4
+    // Wavefronts per SIMD: a level's window takes about 34 VGPRs, so up to 3
     // levels fit 168 VGPRs (3 per SIMD), 6 levels 256 (2 per SIMD); 8 levels
     // take 1 per SIMD with up to 512 VGPRs instead of spilling.
-    static constexpr int WAVES_PER_SIMD = KH <= 4 ? 3 : (KH <= 6 ? 2 : 1);
+    static constexpr int WAVES_PER_SIMD = KH <= 3 ? 3 : (KH <= 6 ? 2 : 1);
     static constexpr int HALOX = ((2 * KH + kSCPL - 1) / kSCPL) * kSCPL;
     static constexpr int OX = kSEXW - 2 * HALOX;
     // rows of b (and rain) kept per wave: the levels read rows t-2KH+1 ..
@@ -252,18 +252,23 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
         for (int c = 0; c < kSCPL; ++c) s_b[wave][s][lx + c] = 0.0f;
     float FS[KH + 1][3][4][kSCPL];
     float DS[KH][3][kSCPL];
+    // (h(a) - h(a+1)) * K of each level's last flux row a: the next row's -y
+    // product is its negation (fl(x - y) = -fl(y - x) exactly)
+    float GY[KH + 1][kSCPL];
     float PB[2][2][kSCPL];   // [slot][b, d][column]
     float PF[2][4][kSCPL];   // [slot][channel][column]
 #pragma unroll
     for (int j = 0; j <= KH; ++j)
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < kSCPL; ++c) {
+            GY[j][c] = 0.0f;
 #pragma unroll
-            for (int c = 0; c < kSCPL; ++c) {
+            for (int r = 0; r < 3; ++r) {
 #pragma unroll
                 for (int ch = 0; ch < 4; ++ch) FS[j][r][ch][c] = 0.0f;
                 if (j < KH) DS[j][r][c] = 0.0f;
             }
+        }
     // Rows relative to row-step t = t_begin (phase 0): level 0 holds rows
     // t-1 (slot 2) and t (slot 0); PF rows t, t+1, t+2 (slots 0, 1, 2); PB rows
     // t+1 (slot 1) and t+2 (slot 0).
@@ -273,6 +278,13 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
         accept(t_begin - 1, b, d, DS[0][2]);
         load_bd(g, parity, t_begin, xc, b, d);
         accept(t_begin, b, d, DS[0][0]);
+    }
+    // level 1's product for row t_begin - 1, which its first row reads
+    {
+        const int rm = (t_begin - 1) & (kRing - 1), ro = t_begin & (kRing - 1);
+#pragma unroll
+        for (int c = 0; c < kSCPL; ++c)
+            GY[1][c] = ((DS[0][2][c] + s_b[wave][rm][lx + c]) - (DS[0][0][c] + s_b[wave][ro][lx + c])) * K;
     }
     load_f(g, parity, t_begin, xc, PF[0]);
     load_f(g, parity, t_begin + 1, xc, PF[1]);
@@ -296,16 +308,24 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
             const int a = t - 2 * (j - 1);
             // level j-1 at rows a-1 (m), a (o), a+1 (p); its own flux of row a
             const int m = slot_of(xj + 2 - 3, 3), o = slot_of(xj + 2 - 2, 3), pp = slot_of(xj + 2 - 1, 3);
-            // heads of level j-1 at rows a-1, a, a+1: its depth plus b
-            float hm1[kSCPL], h0[kSCPL], hp1[kSCPL];
+            // heads of level j-1 at rows a, a+1: its depth plus b
+            float h0[kSCPL], hp1[kSCPL];
             {
-                const int rm = (a - 1) & (kRing - 1), ro = a & (kRing - 1), rp = (a + 1) & (kRing - 1);
+                const int ro = a & (kRing - 1), rp = (a + 1) & (kRing - 1);
 #pragma unroll
                 for (int c = 0; c < kSCPL; ++c) {
-                    hm1[c] = DS[j - 1][m][c] + s_b[wave][rm][lx + c];
                     h0[c] = DS[j - 1][o][c] + s_b[wave][ro][lx + c];
                     hp1[c] = DS[j - 1][pp][c] + s_b[wave][rp][lx + c];
                 }
+            }
+            // y products: +y of row a, and -y as the negated +y of row a-1
+            // (kept from the previous row-step, also while the level ramps up)
+            float py[kSCPL], pym[kSCPL];
+#pragma unroll
+            for (int c = 0; c < kSCPL; ++c) {
+                pym[c] = -GY[j][c];
+                py[c] = (h0[c] - hp1[c]) * K;
+                GY[j][c] = py[c];
             }
             const float (&dOwn)[kSCPL] = DS[j - 1][o];
             const float (&dBase)[kSCPL] = DS[j - 1][m];
@@ -316,18 +336,23 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
             // levels start one after the other: level j's first row-step
             // that can reach the output is t_begin + 4 (j - 1)
             if (j > 1 && t < t_begin + 4 * (j - 1)) continue;
-            // flux of row a: flowUpdate.comp:34-62
+            // flux of row a: flowUpdate.comp:34-62.  x products: +x of column
+            // c, whose negation is -x of column c+1 (the last column's from
+            // the right lane's first head; the first column's -x from the
+            // left lane's last +x product).  Lanes at the strip's edges take
+            // their own head as the missing neighbour (zero difference).
             {
-                const float h_left = from_left_lane_or(h0[kSCPL - 1], h0[0]);
                 const float h_right = from_right_lane_or(h0[0], h0[kSCPL - 1]);
+                float px[kSCPL];
+#pragma unroll
+                for (int c = 0; c < kSCPL; ++c)
+                    px[c] = (h0[c] - (c < kSCPL - 1 ? h0[c + 1] : h_right)) * K;
+                const float pl = from_left_lane_or(px[kSCPL - 1], 0.0f);
 #pragma unroll
                 for (int c = 0; c < kSCPL; ++c) {
-                    const float own = h0[c];
-                    const float hx1 = c < kSCPL - 1 ? h0[c + 1] : h_right;
-                    const float hx0 = c > 0 ? h0[c - 1] : h_left;
                     float o0 = fOwn[0][c], o1 = fOwn[1][c], o2 = fOwn[2][c], o3 = fOwn[3][c];
-                    flux_update(F, K, C, dOwn[c], own - hx1, own - hx0, own - hp1[c],
-                                own - hm1[c], o0, o1, o2, o3);
+                    flux_update_p(F, C, dOwn[c], px[c], -(c > 0 ? px[c - 1] : pl), py[c],
+                                  pym[c], o0, o1, o2, o3);
                     FN[0][c] = o0; FN[1][c] = o1; FN[2][c] = o2; FN[3][c] = o3;
                 }
             }

@@ -256,6 +256,7 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
     // product is its negation (fl(x - y) = -fl(y - x) exactly)
     float GY[KH + 1][kSCPL];
     float PB[2][2][kSCPL];   // [slot][b, d][column]
+    float BR[2][kSCPL];      // terrain of the last two input rows (level 1 reads them)
     float PF[2][4][kSCPL];   // [slot][channel][column]
 #pragma unroll
     for (int j = 0; j <= KH; ++j)
@@ -278,6 +279,8 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
         accept(t_begin - 1, b, d, DS[0][2]);
         load_bd(g, parity, t_begin, xc, b, d);
         accept(t_begin, b, d, DS[0][0]);
+#pragma unroll
+        for (int c = 0; c < kSCPL; ++c) BR[0][c] = b[c];
     }
     // level 1's product for row t_begin - 1, which its first row reads
     {
@@ -297,25 +300,50 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
         // level 0: input row t+1, then the loads of row t+3
         accept(t + 1, PB[slot_of(P + 1, 2)][0], PB[slot_of(P + 1, 2)][1],
                DS[0][slot_of(P + 1, 3)]);
+#pragma unroll
+        for (int c = 0; c < kSCPL; ++c) BR[slot_of(P + 1, 2)][c] = PB[slot_of(P + 1, 2)][0][c];
         load_bd(g, parity, t + 3, xc, PB[slot_of(P + 1, 2)][0], PB[slot_of(P + 1, 2)][1]);
         // input flux of row t, for level 1
         f_fix(g, t, x0, inner_f, PF[slot_of(P, 2)]);
 
         float dLast[kSCPL] = {0.0f, 0.0f};  // the last level's new depth row
+        // terrain of the rows a, a+1 the next level reads: level 1's rows t,
+        // t+1 from registers (the last two input rows), each later level's
+        // read from the ring while the level before it computes
+        float bn[2][kSCPL];
+#pragma unroll
+        for (int c = 0; c < kSCPL; ++c) {
+            bn[0][c] = BR[slot_of(P, 2)][c];
+            bn[1][c] = BR[slot_of(P + 1, 2)][c];
+        }
 #pragma unroll
         for (int j = 1; j <= KH; ++j) {
             const int xj = P - 2 * (j - 1);          // phase of the level's flux row a
             const int a = t - 2 * (j - 1);
             // level j-1 at rows a-1 (m), a (o), a+1 (p); its own flux of row a
             const int m = slot_of(xj + 2 - 3, 3), o = slot_of(xj + 2 - 2, 3), pp = slot_of(xj + 2 - 1, 3);
-            // heads of level j-1 at rows a, a+1: its depth plus b
-            float h0[kSCPL], hp1[kSCPL];
-            {
+            // heads of level j-1 at rows a, a+1: its depth plus b (the rain
+            // kernel, short of registers, reads b from the ring here)
+            if (RAIN && j > 1) {
                 const int ro = a & (kRing - 1), rp = (a + 1) & (kRing - 1);
 #pragma unroll
                 for (int c = 0; c < kSCPL; ++c) {
-                    h0[c] = DS[j - 1][o][c] + s_b[wave][ro][lx + c];
-                    hp1[c] = DS[j - 1][pp][c] + s_b[wave][rp][lx + c];
+                    bn[0][c] = s_b[wave][ro][lx + c];
+                    bn[1][c] = s_b[wave][rp][lx + c];
+                }
+            }
+            float h0[kSCPL], hp1[kSCPL];
+#pragma unroll
+            for (int c = 0; c < kSCPL; ++c) {
+                h0[c] = DS[j - 1][o][c] + bn[0][c];
+                hp1[c] = DS[j - 1][pp][c] + bn[1][c];
+            }
+            if (!RAIN && j < KH) {
+                const int ro = (a - 2) & (kRing - 1), rp = (a - 1) & (kRing - 1);
+#pragma unroll
+                for (int c = 0; c < kSCPL; ++c) {
+                    bn[0][c] = s_b[wave][ro][lx + c];
+                    bn[1][c] = s_b[wave][rp][lx + c];
                 }
             }
             // y products: +y of row a, and -y as the negated +y of row a-1

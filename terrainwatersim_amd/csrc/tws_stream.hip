@@ -39,8 +39,6 @@ namespace {
 constexpr int kSCPL = 2;                 // columns per lane
 constexpr int kSEXW = kSCPL * kWave;     // strip width incl. halo
 constexpr int kStreamWaves = 4;          // independent wavefronts per work group
-constexpr int kHwRegHwId = 4;            // hardware register HW_ID
-constexpr int kHwIdWaveBits = 4;         // its bits [3:0]: the wave slot in the SIMD
 
 template <int KH>
 struct StreamGeom {
@@ -450,20 +448,12 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
     // by the order in which the preloads happened to issue.  vmcnt(0), other
     // counters left at their maxima (gfx9 encoding: expcnt 7, lgkmcnt 15).
     __builtin_amdgcn_s_waitcnt(0x0F70);
-    // Even progress of the wavefronts that share a SIMD.  The two wavefronts
-    // of a SIMD come from two work groups with the same work, and when both
-    // are ready the SIMD issues from the older one: measured at 4096^2, the
-    // younger finished 19% later and ran alone (its latencies exposed) for
-    // the last part of the launch.  So priority alternates between them in
-    // epochs of 2^11 ticks of the 100 MHz real-time counter (20.5 us), by
-    // the parity of the wave slot, re-evaluated every three row-steps.
-    const unsigned slot_parity =
-        __builtin_amdgcn_s_getreg(kHwRegHwId | (0 << 6) | ((kHwIdWaveBits - 1) << 11)) & 1u;
-    auto share_simd = [&]() {
-        const unsigned epoch = (unsigned)(__builtin_amdgcn_s_memrealtime() >> 11);
-        if ((epoch ^ slot_parity) & 1u) __builtin_amdgcn_s_setprio(1);
-        else __builtin_amdgcn_s_setprio(0);
-    };
+    // Even progress of the wavefronts that share a SIMD (one from each of
+    // two work groups with the same work): measured at 4096^2, the younger
+    // one finished 19% later without turns.  Epochs of 2^11 ticks (20.5 us),
+    // re-evaluated every three row-steps.
+    const unsigned turn = turn_parity<1>();
+    auto share_simd = [&]() { take_turn<11>(turn); };
     // Six row-steps per iteration: every ring has 2 or 3 slots, so each of
     // the six phases addresses its slots at compile time.  The loop body has
     // no exit (the remaining 0..5 row-steps follow it), so the waits on the

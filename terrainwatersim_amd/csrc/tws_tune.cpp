@@ -7,7 +7,7 @@
 // are bound by the latency of one tile's levels, so more and smaller tiles
 // finish sooner), 128 x 64 tiles beyond (two work groups per CU so one loads
 // while the other computes), 5 steps per launch up to 4096^2 and 4 above;
-// grids above 3072^2 in closed walls take the streaming kernel at k = 6.
+// grids above 2176^2 in closed walls take the streaming kernel at k = 6.
 // tws_autotune() times every candidate on scratch grids of the requested
 // shapes and returns the winner; only tws_tune_record() enters a launch into
 // the per-size-class table (log2 of the cell count) that choose_launch()
@@ -77,12 +77,14 @@ void choose_launch(int width, int height, int* kernel, int* k) {
             return;
         }
     }
-    // Above 3072^2 the streaming kernel at k = 6 leads (closed walls; other
+    // Above 2176^2 the streaming kernel at k = 6 leads (closed walls; other
     // boundaries step on tile_launch's choice).  MI355X, 0.5 m sheet on fBm
-    // terrain in closed walls, cell-updates/s stream vs the tile table: 3072^2 3.36e11 vs 3.33e11,
-    // 4096^2 3.92e11 vs 3.67e11, 6144^2 4.53e11 vs 3.78e11, 8192^2 4.61e11 vs
-    // 3.98e11, 16384^2 5.24e11 vs 4.26e11; 2048^2 2.33e11 vs 2.90e11.
-    if (cells > 3072 * 3072) {
+    // terrain in closed walls, cell-updates/s stream (k = 6) vs the tile
+    // table: 2048^2 2.71e11 (k = 5: 2.89e11) vs 3.02e11, 2304^2 3.19e11
+    // (k = 5: 3.22e11) vs 2.95e11, 2560^2 3.50e11 vs 3.15e11, 2816^2 3.70e11
+    // vs 3.29e11, 3072^2 3.96e11 vs 3.40e11; 4096^2 4.42e11 vs 3.67e11, 8192^2
+    // 4.61e11 vs 3.98e11 and 16384^2 5.3e11 vs 4.26e11 in other runs.
+    if (cells > 2176 * 2176) {
         *kernel = TWS_KERNEL_STREAM_128;
         *k = 6;
         return;

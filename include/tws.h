@@ -345,8 +345,8 @@ int tws_batch_run(tws_batch* batch);
 /* tws_batch_set_steps(n) for every grid, then tws_batch_run. */
 int tws_batch_step(tws_batch* batch, int32_t n);
 /* Kernel variant and steps per launch for the whole batch (as tws_set_launch;
- * the tile kernels only: TWOPASS and STREAM_128 are not available for
- * batches).  Same as tws_batch_set_launch_classes with one class.
+ * TWOPASS is not available for batches).  Same as tws_batch_set_launch_classes
+ * with one class; TWS_KERNEL_AUTO (the default) is 128x64 tiles at k = 4.
  * tws_batch_get_launch reports the class of the largest grids. */
 int tws_batch_set_launch(tws_batch* batch, int32_t kernel, int32_t steps_per_launch);
 int tws_batch_get_launch(const tws_batch* batch, int32_t* kernel, int32_t* steps_per_launch);
@@ -356,8 +356,11 @@ int tws_batch_get_launch(const tws_batch* batch, int32_t* kernel, int32_t* steps
  * (0 <= c < n <= TWS_BATCH_MAX_CLASSES) steps the grids of at most
  * max_cells[c] cells that no earlier class takes; the last class takes every
  * larger grid (its max_cells entry is ignored).  Bounds must be positive and
- * ascending.  Each class with pending steps gets its own chain of launches;
- * the chains of a run are captured as parallel branches of one hipGraph.
+ * ascending.  Each tile class with pending steps gets its own chain of
+ * launches; the chains of a run are captured as parallel branches of one
+ * hipGraph.  A class on TWS_KERNEL_STREAM_128 steps its grids one after
+ * another with direct launches after the graph (grids it cannot step, such as
+ * those on the reference's edge, take the tile table's launch).
  * Results do not depend on the classes: every launch configuration steps a
  * grid bit-identically.  Has no counterpart in the reference, which steps
  * one terrain at a time. */

@@ -11,6 +11,9 @@
 // grids) and are captured as parallel branches of one hipGraph.  The chains
 // read their tables at run time, so a captured graph of a shape (per class:
 // kernel, k, launches, block bound, mode) is replayed for any table that fits it.
+// A class on the streaming kernel steps its grids one after another with
+// direct launches after the tile chains (each launch fills the device on its
+// own).
 
 #include <hip/hip_runtime.h>
 
@@ -70,7 +73,7 @@ struct tws_batch {
     GridDesc* table_host = nullptr;  // pinned
     int* hdr_host = nullptr;         // pinned
     hipEvent_t table_copied = nullptr;
-    std::vector<LaunchClass> classes{{INT64_MAX, TWS_KERNEL_TILE_128x64, 4}};
+    std::vector<LaunchClass> classes;
     hipStream_t aux[TWS_BATCH_MAX_CLASSES - 1] = {};  // branch streams of the chains
     hipEvent_t fork = nullptr;
     hipEvent_t join[TWS_BATCH_MAX_CLASSES - 1] = {};
@@ -215,6 +218,8 @@ int tws_batch_create(int32_t max_grids, int64_t pool_bytes, void* stream, tws_ba
     if (e == hipSuccess) e = hipEventRecord(b->table_copied, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) { tws_batch_destroy(b); return hip_fail(e, "tws_batch_create: buffers"); }
+    int rc = tws_batch_set_launch(b, TWS_KERNEL_AUTO, 0);
+    if (rc != TWS_OK) { tws_batch_destroy(b); return rc; }
     *out = b;
     return TWS_OK;
 }
@@ -368,12 +373,13 @@ int tws_batch_run(tws_batch* b) {
     TWS_HIP(hipEventSynchronize(b->table_copied), "tws_batch_run: wait table");
     const int nc = (int)b->classes.size();
     int n = 0;
+    bool streaming = false;
     GraphKey chains;
     for (int c = 0; c < nc; ++c) {
         const LaunchClass& lc = b->classes[c];
         const int first = n;
         int tiles = 0, max_steps = 0, mode = kModePlain;
-        for (int i = 0; i < b->max_grids; ++i) {
+        for (int i = 0; i < b->max_grids && lc.kernel != TWS_KERNEL_STREAM_128; ++i) {
             Slot& s = b->slots[i];
             if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c) continue;
             GridDesc g = s.g.desc();
@@ -386,6 +392,12 @@ int tws_batch_run(tws_batch* b) {
             max_steps = std::max(max_steps, s.steps);
             b->table_host[n++] = g;
         }
+        if (lc.kernel == TWS_KERNEL_STREAM_128) {
+            for (int i = 0; i < b->max_grids && !streaming; ++i) {
+                const Slot& s = b->slots[i];
+                streaming = s.used && s.steps > 0 && class_of(b, (int64_t)s.g.W * s.g.H) == c;
+            }
+        }
         b->hdr_host[kHdrInts * c + 0] = n - first;
         b->hdr_host[kHdrInts * c + 1] = tiles;
         b->hdr_host[kHdrInts * c + 2] = first;
@@ -395,14 +407,31 @@ int tws_batch_run(tws_batch* b) {
         chains.push_back(Chain{c, lc.kernel, lc.k, (max_steps + lc.k - 1) / lc.k,
                                (tiles + 511) / 512 * 512, mode});
     }
-    if (n == 0) return TWS_OK;
-    TWS_HIP(hipMemcpyAsync(b->table_dev, b->table_host, sizeof(GridDesc) * (size_t)n,
-                           hipMemcpyHostToDevice, b->stream), "tws_batch_run: table");
-    TWS_HIP(hipMemcpyAsync(b->hdr_dev, b->hdr_host, sizeof(int) * kHdrInts * (size_t)nc,
-                           hipMemcpyHostToDevice, b->stream), "tws_batch_run: header");
+    if (n == 0 && !streaming) return TWS_OK;
+    if (n > 0) {
+        TWS_HIP(hipMemcpyAsync(b->table_dev, b->table_host, sizeof(GridDesc) * (size_t)n,
+                               hipMemcpyHostToDevice, b->stream), "tws_batch_run: table");
+        TWS_HIP(hipMemcpyAsync(b->hdr_dev, b->hdr_host, sizeof(int) * kHdrInts * (size_t)nc,
+                               hipMemcpyHostToDevice, b->stream), "tws_batch_run: header");
+    }
     TWS_HIP(hipEventRecord(b->table_copied, b->stream), "tws_batch_run: event");
-    int rc = run_launches(b, chains);
-    if (rc != TWS_OK) return rc;
+    if (!chains.empty()) {
+        const int rc = run_launches(b, chains);
+        if (rc != TWS_OK) return rc;
+    }
+    // streaming classes after the tile chains, on the batch stream: each of
+    // their launches takes the whole device
+    for (int c = 0; c < nc && streaming; ++c) {
+        const LaunchClass& lc = b->classes[c];
+        if (lc.kernel != TWS_KERNEL_STREAM_128) continue;
+        for (int i = 0; i < b->max_grids; ++i) {
+            Slot& s = b->slots[i];
+            if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c) continue;
+            const int rc = step_grid(s.g, lc.kernel, lc.k, s.steps, b->stream, nullptr, 0, nullptr);
+            if (rc != TWS_OK) return rc;
+            s.steps = 0;  // stepped (step_grid keeps the grid's parity)
+        }
+    }
     for (int i = 0; i < b->max_grids; ++i) {
         Slot& s = b->slots[i];
         if (!s.used || s.steps <= 0) continue;
@@ -424,6 +453,11 @@ int tws_batch_step(tws_batch* b, int32_t n) {
 int tws_batch_set_launch(tws_batch* b, int32_t kernel, int32_t k) {
     if (!b) return fail(TWS_ERR_INVALID, "tws_batch_set_launch: NULL handle");
     if (kernel == TWS_KERNEL_AUTO) {
+        // 128x64 tiles at k = 4 for every grid: the autotuner's choice for the
+        // batched mix and for each size class of it.  A class on the
+        // streaming kernel for the grids above 2176^2 (as choose_launch steps
+        // them alone) lowered the served mix's capacity from 3.78e11 to
+        // 3.35-3.38e11 cell-updates/s, run beside the tile chains or after them.
         kernel = TWS_KERNEL_TILE_128x64;
         k = 4;
     }
@@ -447,7 +481,11 @@ int tws_batch_set_launch_classes(tws_batch* b, int32_t n, const int64_t* max_cel
                     TWS_BATCH_MAX_CLASSES);
     for (int c = 0; c < n; ++c) {
         TileShape sh{};
-        if (!tile_shape(kernels[c], ks[c], &sh))
+        int ox = 0;
+        const bool ok = kernels[c] == TWS_KERNEL_STREAM_128
+                            ? ks[c] >= 2 && stream_shape(ks[c], &ox)
+                            : tile_shape(kernels[c], ks[c], &sh);
+        if (!ok)
             return fail(TWS_ERR_INVALID, "tws_batch_set_launch_classes: class %d: kernel %d / k %d "
                         "not available for batches", c, kernels[c], ks[c]);
         if (c + 1 < n && (max_cells[c] <= 0 || (c > 0 && max_cells[c] <= max_cells[c - 1])))
@@ -464,6 +502,7 @@ int tws_batch_set_launch_classes(tws_batch* b, int32_t n, const int64_t* max_cel
     if (n > 1 && !b->fork)
         TWS_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming),
                 "tws_batch_set_launch_classes: event");
+
     std::vector<LaunchClass> cl;
     try {
         for (int c = 0; c < n; ++c)

@@ -44,6 +44,11 @@ struct GridState {
 };
 
 int grid_clear(GridState& g, hipStream_t s);
+// n steps of one grid on a kernel variant (tws_sim.hip): the streaming kernel
+// where it applies (else the tile table's launch), tile launches of k steps,
+// or the two-pass kernels; optional events around each launch.
+int step_grid(GridState& gs, int kernel, int k, int32_t n, hipStream_t stream, hipEvent_t* ev,
+              int ev_pairs, int* launches);
 int grid_upload(GridState& g, const float* terrain, const float* depth, const float* flux4,
                 hipStream_t s);
 int grid_download(GridState& g, float* depth, float* flux4, float* velocity2, hipStream_t s);

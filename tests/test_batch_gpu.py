@@ -278,13 +278,60 @@ def test_launch_classes_bitwise(gpu, oracle, graphs):
 
 
 def test_launch_classes_rejects(gpu):
+    """Invalid class lists leave the classes as they were; the default (and
+    TWS_KERNEL_AUTO) is one class, 128x64 tiles at k = 4."""
     with gpu.Batch(4, 64 << 20) as batch:
-        T = gpu.KERNEL_TILE_128x64
+        T, S = gpu.KERNEL_TILE_128x64, gpu.KERNEL_STREAM_128
+        default = [(2**63 - 1, T, 4)]
+        assert batch.get_launch_classes() == default
+        assert batch.get_launch() == (T, 4)
         for bad in ([], [(0, T, 4)] * 5, [(500, T, 4), (400, T, 4), (0, T, 4)],
-                    [(0, T, 4), (0, T, 4)], [(0, gpu.KERNEL_STREAM_128, 6)], [(0, T, 7)],
+                    [(0, T, 4), (0, T, 4)], [(0, S, 7)], [(0, S, 1)], [(0, T, 7)],
                     [(0, gpu.KERNEL_TWOPASS, 1)]):
             with pytest.raises(gpu.TwsError):
                 batch.set_launch_classes(bad)
-        assert batch.get_launch() == (T, 4)
+        assert batch.get_launch_classes() == default
         batch.set_launch(gpu.KERNEL_TILE_64x64, 5)
         assert batch.get_launch_classes() == [(2**63 - 1, gpu.KERNEL_TILE_64x64, 5)]
+        batch.set_launch(gpu.KERNEL_AUTO, 0)
+        assert batch.get_launch_classes() == default
+
+
+STREAM_CLASS_SHAPES = [(1, 1), (90, 110), (250, 240), (520, 700), (640, 300), (301, 1033)]
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_launch_classes_with_stream_bitwise(gpu, oracle, graphs):
+    """A batch whose largest grids fall into a class on the streaming kernel
+    (stepped with direct launches after the replayed tile chains), a grid on
+    the reference's edge in that class (it steps on the tile table's launch),
+    per-grid step counts: three runs, each grid after each run bit-identical
+    to the FP32 restatement."""
+    classes = [(300 * 300, gpu.KERNEL_TILE_64x64_16W, 3), (0, gpu.KERNEL_STREAM_128, 5)]
+    grids = []
+    for i, (H, W) in enumerate(STREAM_CLASS_SHAPES):
+        b, d, f = state(500 + i, H, W)
+        boundary = gpu.BOUNDARY_ZERO if i == 4 else gpu.BOUNDARY_CLOSED
+        p = gpu.params_physical(0.02 + 0.002 * i, 9.81, 1.0, 1.0, 0.995, boundary)
+        rain = (W / 2.0, H / 3.0, 1e-2, 64.0) if i % 2 == 1 else None
+        grids.append((b, d, f, p, rain))
+    steps = [4, 9, 7, 13, 6, 11]
+    oracle.set_threads(16)
+    want = [(d.copy(), f.copy()) for _, d, f, _, _ in grids]
+    with gpu.Batch(8, 1 * GB) as batch:
+        batch.use_graphs(graphs)
+        batch.set_launch_classes(classes)
+        ids = load_batch(gpu, batch, grids)
+        for run in range(3):
+            for gid, n in zip(ids, steps):
+                batch.set_steps(gid, n)
+            batch.run()
+            for j, ((b, _, _, p, rain), gid, n) in enumerate(zip(grids, ids, steps)):
+                F, K, Cc, bd, _ = p.as_tuple()
+                d32, f32, _ = oracle.step(b, want[j][0], want[j][1], F, K, Cc, n=n, boundary=bd,
+                                          rain=rain)
+                want[j] = (d32, f32)
+                out = batch.download(gid, flux=True)
+                assert np.array_equal(out["depth"], d32), (run, STREAM_CLASS_SHAPES[j])
+                assert np.array_equal(out["flux"], f32), (run, STREAM_CLASS_SHAPES[j])
+                assert batch.check_finite(gid)

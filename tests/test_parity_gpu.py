@@ -162,7 +162,9 @@ def test_config1_rain_1000_steps(gpu, oracle):
 def test_full_size_4096_properties(gpu, oracle):
     """4096^2, uniform 0.5 m sheet on fBm terrain (the HBM-bound workload):
     first 2 steps bitwise vs the oracle; 1000 steps conserve volume and keep
-    d >= 0; blocked k=4 equals unblocked k=1 after 200 steps."""
+    d >= 0; after 200 steps the default launch (the streaming kernel at k =
+    6), the tile kernel's launch for this size (128x64, k = 5) and 128x128 at
+    k = 4 all equal unblocked 128x128 tiles at k = 1."""
     N = 4096
     b = gpu.generate_fbm_terrain(N, N, seed=2)
     d = np.full((N, N), 0.5, F32)
@@ -175,13 +177,15 @@ def test_full_size_4096_properties(gpu, oracle):
     de, fe, _ = oracle.step(b, d, None, *p3(p), n=2)
     assert_same(out, de, fe)
     res = {}
-    for kernel, k in ((2, 4), (2, 1)):
+    for kernel, k in ((2, 1), (0, 0), (3, 5), (2, 4)):
         with gpu.Simulation(N, N, p) as s:
-            s.set_launch(kernel, k)
+            if kernel:
+                s.set_launch(kernel, k)
             s.upload(b, d)
             s.step(200)
-            res[k] = s.download(flux=True)
-    assert_same(res[4], res[1]["depth"], res[1]["flux"])
+            res[(kernel, k)] = s.download(flux=True)
+    for key in ((0, 0), (3, 5), (2, 4)):
+        assert_same(res[key], res[(2, 1)]["depth"], res[(2, 1)]["flux"])
     with gpu.Simulation(N, N, p) as s:
         s.upload(b, d)
         s.step(1000)

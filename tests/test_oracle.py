@@ -8,10 +8,16 @@ equivariance), an independent whole-array numpy restatement compared bit for
 bit, and FP32-vs-FP64 agreement.  The llvmpipe golden fixtures of the
 unmodified shaders are checked in test_golden.py.
 """
+import os
+import shutil
+import subprocess
 from fractions import Fraction
+from pathlib import Path
 
 import numpy as np
 import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
 
 F32 = np.float32
 
@@ -270,3 +276,70 @@ def test_dispatch_mode_freezes_the_uncovered_tail(oracle):
         inner = (slice(8, uh), slice(8, uw))
         assert not f1[inner].any() and (d1[inner] == 1.0).all()
         assert f1[:2, :uw].any() and (d1[0, :uw] < 1.0).all()
+
+
+SANITIZED_DRIVER = r"""
+#include <stdio.h>
+#include <stdlib.h>
+int oracle_step_f32(int W, int H, const float* b, float* d, float* f, float F, float K, float C,
+                    int boundary, float ghost_alpha, const float* rain, int nsteps, float* v);
+int oracle_step_f64(int W, int H, const double* b, double* d, double* f, double F, double K,
+                    double C, int boundary, double ghost_alpha, const float* rain, int nsteps,
+                    double* v);
+int oracle_mip_levels(int W, int H);
+void oracle_mips(int W, int H, const float* rgba0, float* out);
+void oracle_set_threads(int n);
+static float frand(unsigned* s) { *s = *s * 1664525u + 1013904223u; return (*s >> 8) / 16777216.0f; }
+int main(void) {
+    const int shapes[][2] = {{1, 1}, {1, 7}, {5, 1}, {2, 3}, {17, 33}, {40, 33}, {70, 90}};
+    const float rain[4] = {3.5f, 2.25f, 1e-2f, 64.0f};
+    unsigned seed = 1;
+    oracle_set_threads(2);
+    for (unsigned s = 0; s < sizeof shapes / sizeof shapes[0]; ++s) {
+        const int H = shapes[s][0], W = shapes[s][1];
+        const size_t n = (size_t)W * H;
+        float *b = malloc(n * 4), *d = malloc(n * 4), *f = malloc(n * 16), *v = malloc(n * 8);
+        double *bd = malloc(n * 8), *dd = malloc(n * 8), *fd = malloc(n * 32), *vd = malloc(n * 16);
+        for (int boundary = 0; boundary < 3; ++boundary) {
+            for (size_t i = 0; i < n; ++i) {
+                b[i] = 4 * frand(&seed); d[i] = 2 * frand(&seed); bd[i] = b[i]; dd[i] = d[i];
+                for (int c = 0; c < 4; ++c) { f[4 * i + c] = 0.3f * frand(&seed); fd[4 * i + c] = f[4 * i + c]; }
+            }
+            if (oracle_step_f32(W, H, b, d, f, 0.99f, 0.1962f, 0.02f, boundary, 0.0f, rain, 9, v) ||
+                oracle_step_f64(W, H, bd, dd, fd, 0.99, 0.1962, 0.02, boundary, 0.0, rain, 9, vd)) {
+                printf("step failed\n"); return 1;
+            }
+        }
+        const int levels = oracle_mip_levels(W, H);
+        size_t total = 0;
+        for (int l = 1, w = W, h = H; l < levels; ++l) {
+            w = w > 1 ? w / 2 : 1; h = h > 1 ? h / 2 : 1; total += (size_t)w * h * 4;
+        }
+        float *rgba = malloc(n * 16), *mips = malloc((total ? total : 1) * 4);
+        for (size_t i = 0; i < n * 4; ++i) rgba[i] = frand(&seed);
+        oracle_mips(W, H, rgba, mips);
+        free(rgba); free(mips);
+        free(b); free(d); free(f); free(v); free(bd); free(dd); free(fd); free(vd);
+    }
+    printf("OK\n");
+    return 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_oracle_under_sanitizers(tmp_path):
+    """The C restatement built with AddressSanitizer and UndefinedBehaviorSanitizer
+    (host code only): FP32 and FP64 steps with rain on degenerate and ragged
+    shapes in every edge mode, and the mip chain, without a report."""
+    src = tmp_path / "driver.c"
+    src.write_text(SANITIZED_DRIVER)
+    exe = tmp_path / "driver"
+    odir = ROOT / "oracle"
+    subprocess.run(["gcc", "-O1", "-g", "-std=c11", "-ffp-contract=off", "-fopenmp",
+                    "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+                    f"-I{odir}", str(odir / "tws_oracle.c"), str(src), "-o", str(exe), "-lm"],
+                   check=True)
+    env = dict(os.environ, OMP_NUM_THREADS="2", ASAN_OPTIONS="detect_leaks=1")
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300, env=env)
+    assert out.returncode == 0 and out.stdout.strip() == "OK", out.stdout + out.stderr[-3000:]

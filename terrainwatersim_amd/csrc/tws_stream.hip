@@ -7,7 +7,8 @@
 // flux of row a = t - 2(j-1) (flowUpdate.comp:34-62) and the depth of row
 // a-1 (flowApply.comp:32-41), which needs level j-1 only at rows a-1..a+1,
 // all of them finished by then.  So each level holds a window of three rows
-// of depth/head and two of flux in VGPRs, and
+// of depth and three of flux in VGPRs (heads are recomputed from the terrain
+// as they are read), and
 //   - nothing is recomputed in y except the 2 KH rows a segment's cone
 //     reaches into its neighbours (the tile kernels recompute 2 KH rows at
 //     both ends of every 64-row tile);
@@ -16,7 +17,10 @@
 //     own window;
 //   - terrain b, read by each level at its own rows, sits in a wave-private
 //     ring of 16 (32 for 8 levels) rows in LDS, the rain amounts beside it;
-//   - the next input row is loaded one row-step ahead;
+//     a level's rows are read while the level before it computes;
+//   - each head difference serves the two cells it separates (negated for
+//     the second: fl(a - b) = -fl(b - a));
+//   - the input rows are loaded two row-steps ahead;
 //   - the two wavefronts sharing a SIMD take turns at the higher issue
 //     priority, so neither runs alone at the end of the launch.
 // The x-halo is the tile kernels': HALOX = 2 KH columns on each side go stale
@@ -44,7 +48,8 @@ template <int KH>
 struct StreamGeom {
     // Wavefronts per SIMD: a level's window takes about 34 VGPRs, so up to 3
     // levels fit 168 VGPRs (3 per SIMD), 6 levels 256 (2 per SIMD); 8 levels
-    // take 1 per SIMD with up to 512 VGPRs instead of spilling.
+    // take 1 per SIMD, and past the 256 architectural VGPRs of a wavefront
+    // the compiler keeps values in accumulation registers.
     static constexpr int WAVES_PER_SIMD = KH <= 3 ? 3 : (KH <= 6 ? 2 : 1);
     static constexpr int HALOX = ((2 * KH + kSCPL - 1) / kSCPL) * kSCPL;
     static constexpr int OX = kSEXW - 2 * HALOX;
@@ -157,12 +162,13 @@ __device__ __forceinline__ void f_fix(const GridDesc& g, int y, int x0, bool inn
 // (MODE kModePlain or kModeRain).
 //
 // Registers, per lane and column: level j >= 1 keeps the flux of its rows
-// a, a-1, a-2 in a 3-slot ring FS[j] (the new row a overwrites a-3), levels
-// 0..KH-1 the depth of their last three output rows in DS[j] (heads are
-// recomputed from the b ring as the next level reads them: 6 VGPRs per level
-// fewer, which with the rest lets 4 levels run 3 wavefronts per SIMD); the
-// input rows come through PB (b and d, two rows ahead) and PF (flux, the row
-// level 1 reads plus one ahead).
+// a, a-1, a-2 in a 3-slot ring FS[j] (the new row a overwrites a-3) and the
+// +y product of its last flux row in GY[j], levels 0..KH-1 the depth of
+// their last three output rows in DS[j] (heads are recomputed from the b
+// ring as the next level reads them: 6 VGPRs per level fewer); the input
+// rows come through PB (b and d, two rows ahead), BR (the terrain of the last
+// two input rows, which level 1 reads) and PF (flux, the row level 1 reads
+// plus one ahead).
 template <int KH, int MODE>
 __global__ void __launch_bounds__(kStreamWaves * kWave, StreamGeom<KH>::WAVES_PER_SIMD)
 k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {

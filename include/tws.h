@@ -270,13 +270,17 @@ int tws_volume_noalloc(const float* depth, int32_t width, int32_t height, int64_
                                        a 128-column strip segment down once
                                        per launch with all k levels in a
                                        register window, so only the x-halo is
-                                       recomputed; single grids with closed
-                                       walls (others step on the default
-                                       tile kernel); k = 2, 3, 4, 5, 6 or 8 */
+                                       recomputed; closed walls (grids on
+                                       other edges, or with planes of 2 GiB or
+                                       more, step on the tile kernel's
+                                       launch); k = 2, 3, 4, 5, 6 or 8; also a
+                                       batch size class */
 #define TWS_KERNEL_COUNT 9
 
 /* Selects a kernel variant and the number of steps per launch (1, 2, 3, 4,
- * 5, 6 or 8; TWOPASS requires 1).  TWS_KERNEL_AUTO picks from the tuning table. */
+ * 5, 6 or 8; TWOPASS requires 1, STREAM_128 at least 2).  TWS_KERNEL_AUTO
+ * picks from the tuning table: tile kernels up to 2176^2, STREAM_128 at k = 6
+ * for closed grids above (a launch entered with tws_tune_record first). */
 int tws_set_launch(tws_sim* sim, int32_t kernel, int32_t steps_per_launch);
 int tws_get_launch(const tws_sim* sim, int32_t* kernel, int32_t* steps_per_launch);
 /* Order in which the step kernel walks a grid's tiles: groups of

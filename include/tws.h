@@ -350,21 +350,29 @@ int tws_batch_run(tws_batch* batch);
 int tws_batch_step(tws_batch* batch, int32_t n);
 /* Kernel variant and steps per launch for the whole batch (as tws_set_launch;
  * TWOPASS is not available for batches).  Same as tws_batch_set_launch_classes
- * with one class; TWS_KERNEL_AUTO (the default) is 128x64 tiles at k = 4.
+ * with one class.  TWS_KERNEL_AUTO (the default; reported as kernel
+ * TWS_KERNEL_AUTO, k 0) picks per run: TWS_KERNEL_STREAM_128 at k = 6 when the
+ * grids with pending steps that the streaming kernel can step hold more than
+ * 2^23 cells (2896^2) together, 128x64 tiles at k = 4 otherwise.
  * tws_batch_get_launch reports the class of the largest grids. */
 int tws_batch_set_launch(tws_batch* batch, int32_t kernel, int32_t steps_per_launch);
 int tws_batch_get_launch(const tws_batch* batch, int32_t* kernel, int32_t* steps_per_launch);
 
 #define TWS_BATCH_MAX_CLASSES 4
-/* Size classes, each with its own tile kernel and steps per launch: class c
+/* Size classes, each with its own kernel and steps per launch (or
+ * TWS_KERNEL_AUTO, whose k is ignored and which picks per run as
+ * tws_batch_set_launch describes, among that class's grids): class c
  * (0 <= c < n <= TWS_BATCH_MAX_CLASSES) steps the grids of at most
  * max_cells[c] cells that no earlier class takes; the last class takes every
  * larger grid (its max_cells entry is ignored).  Bounds must be positive and
  * ascending.  Each tile class with pending steps gets its own chain of
  * launches; the chains of a run are captured as parallel branches of one
- * hipGraph.  A class on TWS_KERNEL_STREAM_128 steps its grids one after
- * another with direct launches after the graph (grids it cannot step, such as
- * those on the reference's edge, take the tile table's launch).
+ * hipGraph.  A class on TWS_KERNEL_STREAM_128 has a chain too: the strip
+ * segments of all its grids are the tasks of each launch, with the segment
+ * length planned for the class's grid shapes so the launch ends in whole
+ * rounds.  Steps short of a full launch of k, and grids the streaming kernel
+ * cannot step (such as those on the reference's edge, which take the tile
+ * table's launch), run as direct launches after the graph.
  * Results do not depend on the classes: every launch configuration steps a
  * grid bit-identically.  Has no counterpart in the reference, which steps
  * one terrain at a time. */
@@ -387,7 +395,8 @@ typedef struct tws_tune_entry {
     double cell_updates_per_s;
 } tws_tune_entry;
 
-/* Times every (tile variant, steps per launch) candidate on scratch grids of
+/* Times every (kernel, steps per launch) candidate -- the six tile variants
+ * and TWS_KERNEL_STREAM_128, each at k = 2, 3, 4, 5, 6 and 8 -- on scratch grids of
  * the n given shapes advanced together as one batch (synthetic state: seeded
  * fBm terrain under a 0.5 m water sheet, a rain disc of radius 16 at each
  * grid's centre), `steps` steps per trial after an untimed run of as many, and
@@ -404,7 +413,9 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
 /* Enters (kernel, steps_per_launch) for the size class (log2 of the cell
  * count) of a width x height grid: TWS_KERNEL_AUTO (tws_create,
  * tws_set_launch) picks it for grids of that class from then on in this
- * process.  TWS_ERR_INVALID for an unknown tile variant or k. */
+ * process (a streaming launch steps grids it cannot step on the tile
+ * table's launch, as tws_set_launch does).  TWS_ERR_INVALID for an unknown
+ * kernel or k. */
 int tws_tune_record(int32_t width, int32_t height, int32_t kernel, int32_t steps_per_launch);
 /* Drops every recorded launch: TWS_KERNEL_AUTO follows the built-in table. */
 int tws_tune_forget(void);

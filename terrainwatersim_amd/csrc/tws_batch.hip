@@ -11,14 +11,18 @@
 // grids) and are captured as parallel branches of one hipGraph.  The chains
 // read their tables at run time, so a captured graph of a shape (per class:
 // kernel, k, launches, block bound, mode) is replayed for any table that fits it.
-// A class on the streaming kernel steps its grids one after another with
-// direct launches after the tile chains (each launch fills the device on its
-// own).
+// A class on the streaming kernel has a chain of its own too: its grids'
+// strip segments are the tasks of one table (k_stream's batch mode), and the
+// chain advances them in full launches of k steps.  The segment cap is
+// planned for the class's grid shapes (stream_batch_cap) and kept while they
+// stay the same.  A grid's remaining steps (fewer than k), and grids the
+// streaming kernel cannot step, run as direct launches after the graph.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <new>
 #include <tuple>
@@ -40,23 +44,51 @@ struct Slot {
     int steps = 0;  // pending for the next run
 };
 
-// A size class: grids of at most max_cells cells not taken by an earlier class.
+// A size class: grids of at most max_cells cells not taken by an earlier
+// class.  kernel TWS_KERNEL_AUTO (k 0) picks its launch per run
+// (resolve_class).
 struct LaunchClass {
     int64_t max_cells;
     int kernel, k;
 };
 
-// One class's launch chain in a run.
+// One class's launch chain in a run.  A streaming chain also ends in tail
+// launches (bit r of `tails`: one launch of r steps over the grids whose step
+// count leaves r after the full launches; work groups: tail_blocks) and in
+// tile launches for its grids the streaming kernel cannot step (alt_*: the
+// fallback tile launch's count, work groups and mode).
 struct Chain {
-    int cls, kernel, k, launches, blocks, mode;
+    int cls, kernel, k, launches, blocks, mode, tails, tail_blocks;
+    int alt_launches, alt_blocks, alt_mode;
     bool operator<(const Chain& o) const {
-        return std::tie(cls, kernel, k, launches, blocks, mode) <
-               std::tie(o.cls, o.kernel, o.k, o.launches, o.blocks, o.mode);
+        return std::tie(cls, kernel, k, launches, blocks, mode, tails, tail_blocks, alt_launches,
+                        alt_blocks, alt_mode) <
+               std::tie(o.cls, o.kernel, o.k, o.launches, o.blocks, o.mode, o.tails,
+                        o.tail_blocks, o.alt_launches, o.alt_blocks, o.alt_mode);
     }
 };
 using GraphKey = std::vector<Chain>;
 
 constexpr int kHdrInts = 4;  // per class: grids, tiles, table offset, unused
+
+// Tail launches of a streaming chain run the kernel's built step counts below
+// 7; a remainder of 7 (k = 8) steps on direct launches.  A streaming class's
+// grids that the streaming kernel cannot step (edges other than closed walls,
+// planes of 2 GiB or more) take the fallback tile launch in the same chain.
+// Headers: the classes' own, kMaxTail per class for its tails, one per class
+// for its fallback table.
+constexpr int kMaxTail = 6;
+constexpr int kAltKernel = TWS_KERNEL_TILE_128x64, kAltK = 4;
+constexpr int kHdrCount = TWS_BATCH_MAX_CLASSES * (2 + kMaxTail);
+inline int tail_hdr(int cls, int r) { return TWS_BATCH_MAX_CLASSES + cls * kMaxTail + (r - 1); }
+inline int alt_hdr(int cls) { return TWS_BATCH_MAX_CLASSES * (1 + kMaxTail) + cls; }
+
+// A streaming class's segment cap, kept while the shapes it was planned for
+// (W, H of the chain's grids in table order) and k stay the same.
+struct CapPlan {
+    std::vector<int> shapes;
+    int k = 0, cap = 0;
+};
 
 }  // namespace
 
@@ -66,14 +98,15 @@ struct tws_batch {
     DevicePool pool;
     int max_grids = 0;
     std::vector<Slot> slots;
-    GridDesc* table_dev = nullptr;   // max_grids entries
-    int* hdr_dev = nullptr;          // {grids, tiles}
+    GridDesc* table_dev = nullptr;   // 2 max_grids entries (a grid and its tail)
+    int* hdr_dev = nullptr;          // kHdrCount headers {grids, tiles, offset, 0}
     unsigned* status_dev = nullptr;  // one word per slot
     double* partials_dev = nullptr;
     GridDesc* table_host = nullptr;  // pinned
     int* hdr_host = nullptr;         // pinned
     hipEvent_t table_copied = nullptr;
     std::vector<LaunchClass> classes;
+    CapPlan caps[TWS_BATCH_MAX_CLASSES];
     hipStream_t aux[TWS_BATCH_MAX_CLASSES - 1] = {};  // branch streams of the chains
     hipEvent_t fork = nullptr;
     hipEvent_t join[TWS_BATCH_MAX_CLASSES - 1] = {};
@@ -121,9 +154,18 @@ static hipError_t issue_chains(tws_batch* b, const GraphKey& chains) {
     for (int c = 0; c < n && e == hipSuccess; ++c) {
         const Chain& ch = chains[c];
         hipStream_t st = c == 0 ? b->stream : b->aux[c - 1];
+        const int* hdr = b->hdr_dev + kHdrInts * ch.cls;
         for (int j = 0; j < ch.launches && e == hipSuccess; ++j)
-            e = launch_tile_batch(ch.kernel, ch.k, ch.mode, b->table_dev, b->hdr_dev + kHdrInts * ch.cls,
-                                  ch.blocks, j, st);
+            e = ch.kernel == TWS_KERNEL_STREAM_128
+                    ? launch_stream_batch(ch.k, ch.mode, b->table_dev, hdr, ch.blocks, j, st)
+                    : launch_tile_batch(ch.kernel, ch.k, ch.mode, b->table_dev, hdr, ch.blocks, j, st);
+        for (int r = 1; r <= kMaxTail && e == hipSuccess; ++r)
+            if ((ch.tails >> r) & 1)
+                e = launch_stream_batch(r, ch.mode, b->table_dev, b->hdr_dev + kHdrInts * tail_hdr(ch.cls, r),
+                                        ch.tail_blocks, 0, st);
+        for (int j = 0; j < ch.alt_launches && e == hipSuccess; ++j)
+            e = launch_tile_batch(kAltKernel, kAltK, ch.alt_mode, b->table_dev,
+                                  b->hdr_dev + kHdrInts * alt_hdr(ch.cls), ch.alt_blocks, j, st);
     }
     for (int c = 1; c < n && e == hipSuccess; ++c) {
         e = hipEventRecord(b->join[c - 1], b->aux[c - 1]);
@@ -207,12 +249,13 @@ int tws_batch_create(int32_t max_grids, int64_t pool_bytes, void* stream, tws_ba
     e = hipMalloc(&base, cap);
     if (e != hipSuccess) { tws_batch_destroy(b); return hip_fail(e, "tws_batch_create: pool"); }
     b->pool.reset(base, cap);
-    e = hipMalloc(&b->table_dev, sizeof(GridDesc) * (size_t)max_grids);
-    if (e == hipSuccess) e = hipMalloc(&b->hdr_dev, 256);
+    e = hipMalloc(&b->table_dev, sizeof(GridDesc) * 2 * (size_t)max_grids);
+    if (e == hipSuccess) e = hipMalloc(&b->hdr_dev, sizeof(int) * kHdrInts * kHdrCount);
     if (e == hipSuccess) e = hipMalloc(&b->status_dev, sizeof(unsigned) * (size_t)max_grids);
     if (e == hipSuccess) e = hipMalloc(&b->partials_dev, sizeof(double) * kVolumeParts);
-    if (e == hipSuccess) e = hipHostMalloc(&b->table_host, sizeof(GridDesc) * (size_t)max_grids);
-    if (e == hipSuccess) e = hipHostMalloc(&b->hdr_host, 256);
+    if (e == hipSuccess) e = hipHostMalloc(&b->table_host, sizeof(GridDesc) * 2 * (size_t)max_grids);
+    if (e == hipSuccess) e = hipHostMalloc(&b->hdr_host, sizeof(int) * kHdrInts * kHdrCount);
+    if (e == hipSuccess) std::memset(b->hdr_host, 0, sizeof(int) * kHdrInts * kHdrCount);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemsetAsync(b->status_dev, 0, sizeof(unsigned) * (size_t)max_grids, b->stream);
     if (e == hipSuccess) e = hipEventRecord(b->table_copied, b->stream);
@@ -367,51 +410,180 @@ int tws_batch_set_steps(tws_batch* b, int32_t id, int32_t steps) {
     return TWS_OK;
 }
 
+// Whether the streaming class's full launches step this grid: the streaming
+// kernel applies and the grid has at least one full launch of k steps.
+static bool in_stream_chain(const Slot& s, int k) { return s.steps >= k && stream_usable(s.g); }
+
+// Whether a grid of a streaming class of k steps per launch ends in a tail
+// launch (rather than a direct launch) for the steps short of a full launch.
+static bool in_stream_tail(const Slot& s, int k) {
+    const int r = s.steps % k;
+    return r > 0 && r <= kMaxTail && stream_usable(s.g);
+}
+
+// The launch class c runs this run.  An AUTO class takes the streaming kernel
+// at k = 6 when its grids with pending steps that the streaming kernel can
+// step hold more than 2^23 cells (2896^2) together, and the fallback tile
+// launch otherwise.  MI355X, ms per 100 steps with a rain disc, streaming
+// k = 6 vs 128x64 tiles at k = 4: 20 x 512^2 2.22 vs 2.01, 1 x 2304^2 2.16
+// vs 1.81; at 2^23 cells 32 x 512^2 2.92 vs 2.87, 8 x 1024^2 2.77 vs 2.86,
+// 2 x 2048^2 2.89 vs 2.97; 1 x 3072^2 2.80 vs 3.06, 1 x 4096^2 4.51 vs 4.87;
+// 256 grids of 128^2..2047^2 4.21e11 vs 3.84e11 cell-updates/s, and the
+// saturated served mix 4.25e11 vs 3.79e11.
+static constexpr int64_t kAutoStreamCells = (int64_t)1 << 23;
+static LaunchClass resolve_class(const tws_batch* b, int c) {
+    LaunchClass lc = b->classes[c];
+    if (lc.kernel != TWS_KERNEL_AUTO) return lc;
+    int64_t cells = 0;
+    for (const Slot& s : b->slots)
+        if (s.used && s.steps > 0 && stream_usable(s.g) &&
+            class_of(b, (int64_t)s.g.W * s.g.H) == c)
+            cells += (int64_t)s.g.W * s.g.H;
+    lc.kernel = cells > kAutoStreamCells ? TWS_KERNEL_STREAM_128 : kAltKernel;
+    lc.k = cells > kAutoStreamCells ? 6 : kAltK;
+    return lc;
+}
+
+// Segment cap of streaming class c's chain in this run (0 if it cannot be
+// planned for lack of host memory).
+static int chain_cap(tws_batch* b, int c, int k) {
+    try {
+        std::vector<int> shapes;
+        for (int i = 0; i < b->max_grids; ++i) {
+            const Slot& s = b->slots[i];
+            if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c ||
+                !in_stream_chain(s, k))
+                continue;
+            shapes.push_back(s.g.W);
+            shapes.push_back(s.g.H);
+        }
+        CapPlan& cp = b->caps[c];
+        if (cp.k != k || cp.cap <= 0 || cp.shapes != shapes) {
+            cp.cap = 0;
+            cp.cap = stream_batch_cap(shapes.data(), (int)shapes.size() / 2, k, device_cus());
+            cp.k = k;
+            cp.shapes.swap(shapes);
+        }
+        return cp.cap;
+    } catch (const std::bad_alloc&) {
+        return 0;
+    }
+}
+
 int tws_batch_run(tws_batch* b) {
     if (!b) return fail(TWS_ERR_INVALID, "tws_batch_run: NULL handle");
     // the pinned table may still be in flight from the previous run
     TWS_HIP(hipEventSynchronize(b->table_copied), "tws_batch_run: wait table");
     const int nc = (int)b->classes.size();
+    LaunchClass run_class[TWS_BATCH_MAX_CLASSES];
+    for (int c = 0; c < nc; ++c) run_class[c] = resolve_class(b, c);
     int n = 0;
-    bool streaming = false;
+    bool direct = false;  // grids of streaming classes with steps outside the chains
     GraphKey chains;
     for (int c = 0; c < nc; ++c) {
-        const LaunchClass& lc = b->classes[c];
+        const LaunchClass& lc = run_class[c];
+        const bool stream = lc.kernel == TWS_KERNEL_STREAM_128;
         const int first = n;
         int tiles = 0, max_steps = 0, mode = kModePlain;
-        for (int i = 0; i < b->max_grids && lc.kernel != TWS_KERNEL_STREAM_128; ++i) {
+        const int cap = stream ? chain_cap(b, c, lc.k) : 0;
+        if (stream && cap <= 0) return fail(TWS_ERR_NOMEM, "tws_batch_run: host allocation failed");
+        for (int i = 0; i < b->max_grids; ++i) {
             Slot& s = b->slots[i];
             if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c) continue;
             GridDesc g = s.g.desc();
-            if (!grid_tiles(s.g.W, s.g.H, lc.kernel, lc.k, &g.tiles_x, &g.tiles_y))
-                return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
-            g.tile_begin = tiles;
-            g.steps = s.steps;
-            mode = std::max(mode, kernel_mode(g));
-            tiles += g.tiles_x * g.tiles_y;
-            max_steps = std::max(max_steps, s.steps);
-            b->table_host[n++] = g;
-        }
-        if (lc.kernel == TWS_KERNEL_STREAM_128) {
-            for (int i = 0; i < b->max_grids && !streaming; ++i) {
-                const Slot& s = b->slots[i];
-                streaming = s.used && s.steps > 0 && class_of(b, (int64_t)s.g.W * s.g.H) == c;
+            if (stream) {
+                if (stream_usable(s.g) && s.steps % lc.k != 0 && !in_stream_tail(s, lc.k))
+                    direct = true;
+                if (!in_stream_chain(s, lc.k)) continue;
+                if (!stream_batch_plan(s.g.W, s.g.H, lc.k, cap, &g.tiles_x, &g.tiles_y, &g.seg_rows))
+                    return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
+                mode = std::max(mode, g.rain_on ? kModeRain : kModePlain);
+                max_steps = std::max(max_steps, s.steps / lc.k * lc.k);  // full launches
+            } else {
+                if (!grid_tiles(s.g.W, s.g.H, lc.kernel, lc.k, &g.tiles_x, &g.tiles_y))
+                    return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
+                mode = std::max(mode, kernel_mode(g));
+                max_steps = std::max(max_steps, s.steps);
             }
+            g.tile_begin = tiles;  // first tile, or first task of the streaming kernel
+            g.steps = s.steps;
+            tiles += g.tiles_x * g.tiles_y;
+            b->table_host[n++] = g;
         }
         b->hdr_host[kHdrInts * c + 0] = n - first;
         b->hdr_host[kHdrInts * c + 1] = tiles;
         b->hdr_host[kHdrInts * c + 2] = first;
         b->hdr_host[kHdrInts * c + 3] = 0;
-        if (n == first) continue;
-        // block bound rounded up so small changes of the tile count reuse a graph
-        chains.push_back(Chain{c, lc.kernel, lc.k, (max_steps + lc.k - 1) / lc.k,
-                               (tiles + 511) / 512 * 512, mode});
+        // block bounds rounded up so small changes of the tile (task) count
+        // reuse a graph; spare blocks exit
+        const int blocks = stream ? ((tiles + 3) / 4 + 63) / 64 * 64 : (tiles + 511) / 512 * 512;
+        // a streaming class's tails: per remainder r, a table of the grids
+        // left with r steps, each entry advanced past the full launches
+        // (parity) and holding exactly r steps, for one launch of r levels
+        int tails = 0, tail_blocks = 0;
+        for (int r = 1; stream && r < lc.k && r <= kMaxTail; ++r) {
+            const int tfirst = n;
+            int tasks = 0;
+            for (int i = 0; i < b->max_grids; ++i) {
+                Slot& s = b->slots[i];
+                if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c ||
+                    s.steps % lc.k != r || !stream_usable(s.g))
+                    continue;
+                GridDesc g = s.g.desc();
+                if (!stream_batch_plan(s.g.W, s.g.H, r, cap, &g.tiles_x, &g.tiles_y, &g.seg_rows))
+                    return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
+                g.parity0 ^= (s.steps / lc.k) & 1;
+                g.steps = r;
+                g.tile_begin = tasks;
+                tasks += g.tiles_x * g.tiles_y;
+                mode = std::max(mode, g.rain_on ? kModeRain : kModePlain);
+                b->table_host[n++] = g;
+            }
+            int* h = b->hdr_host + kHdrInts * tail_hdr(c, r);
+            h[0] = n - tfirst;
+            h[1] = tasks;
+            h[2] = tfirst;
+            h[3] = 0;
+            if (n == tfirst) continue;
+            tails |= 1 << r;
+            tail_blocks = std::max(tail_blocks, ((tasks + 3) / 4 + 63) / 64 * 64);
+        }
+        // a streaming class's grids the streaming kernel cannot step: the
+        // fallback tile launch, as a tile class would step them
+        int alt_steps = 0, alt_tiles = 0, alt_mode = kModePlain;
+        if (stream) {
+            const int afirst = n;
+            for (int i = 0; i < b->max_grids; ++i) {
+                Slot& s = b->slots[i];
+                if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c ||
+                    stream_usable(s.g))
+                    continue;
+                GridDesc g = s.g.desc();
+                if (!grid_tiles(s.g.W, s.g.H, kAltKernel, kAltK, &g.tiles_x, &g.tiles_y))
+                    return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
+                alt_mode = std::max(alt_mode, kernel_mode(g));
+                alt_steps = std::max(alt_steps, s.steps);
+                g.tile_begin = alt_tiles;
+                g.steps = s.steps;
+                alt_tiles += g.tiles_x * g.tiles_y;
+                b->table_host[n++] = g;
+            }
+            int* h = b->hdr_host + kHdrInts * alt_hdr(c);
+            h[0] = n - afirst;
+            h[1] = alt_tiles;
+            h[2] = afirst;
+            h[3] = 0;
+        }
+        if (max_steps == 0 && tails == 0 && alt_steps == 0) continue;
+        chains.push_back(Chain{c, lc.kernel, lc.k, (max_steps + lc.k - 1) / lc.k, blocks, mode,
+                               tails, tail_blocks, (alt_steps + kAltK - 1) / kAltK,
+                               (alt_tiles + 511) / 512 * 512, alt_mode});
     }
-    if (n == 0 && !streaming) return TWS_OK;
+    if (n == 0 && !direct) return TWS_OK;
     if (n > 0) {
         TWS_HIP(hipMemcpyAsync(b->table_dev, b->table_host, sizeof(GridDesc) * (size_t)n,
                                hipMemcpyHostToDevice, b->stream), "tws_batch_run: table");
-        TWS_HIP(hipMemcpyAsync(b->hdr_dev, b->hdr_host, sizeof(int) * kHdrInts * (size_t)nc,
+        TWS_HIP(hipMemcpyAsync(b->hdr_dev, b->hdr_host, sizeof(int) * kHdrInts * kHdrCount,
                                hipMemcpyHostToDevice, b->stream), "tws_batch_run: header");
     }
     TWS_HIP(hipEventRecord(b->table_copied, b->stream), "tws_batch_run: event");
@@ -419,25 +591,32 @@ int tws_batch_run(tws_batch* b) {
         const int rc = run_launches(b, chains);
         if (rc != TWS_OK) return rc;
     }
-    // streaming classes after the tile chains, on the batch stream: each of
-    // their launches takes the whole device
-    for (int c = 0; c < nc && streaming; ++c) {
-        const LaunchClass& lc = b->classes[c];
-        if (lc.kernel != TWS_KERNEL_STREAM_128) continue;
-        for (int i = 0; i < b->max_grids; ++i) {
-            Slot& s = b->slots[i];
-            if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c) continue;
-            const int rc = step_grid(s.g, lc.kernel, lc.k, s.steps, b->stream, nullptr, 0, nullptr);
-            if (rc != TWS_OK) return rc;
-            s.steps = 0;  // stepped (step_grid keeps the grid's parity)
-        }
-    }
     for (int i = 0; i < b->max_grids; ++i) {
         Slot& s = b->slots[i];
         if (!s.used || s.steps <= 0) continue;
-        const int k = b->classes[class_of(b, (int64_t)s.g.W * s.g.H)].k;
-        const int gl = (s.steps + k - 1) / k;
-        s.g.parity ^= (gl & 1);
+        const LaunchClass& lc = run_class[class_of(b, (int64_t)s.g.W * s.g.H)];
+        if (lc.kernel != TWS_KERNEL_STREAM_128) {
+            const int gl = (s.steps + lc.k - 1) / lc.k;
+            s.g.parity ^= (gl & 1);
+        } else {
+            // the chain's full launches and its tail (a remainder of 7 then
+            // on direct launches), or the fallback tile launches
+            int rest = 0;
+            if (stream_usable(s.g)) {
+                s.g.parity ^= ((s.steps / lc.k) & 1);
+                rest = s.steps % lc.k;
+                if (in_stream_tail(s, lc.k)) {
+                    s.g.parity ^= 1;
+                    rest = 0;
+                }
+            } else {
+                s.g.parity ^= ((s.steps + kAltK - 1) / kAltK) & 1;
+            }
+            if (rest > 0) {
+                const int rc = step_grid(s.g, lc.kernel, lc.k, rest, b->stream, nullptr, 0, nullptr);
+                if (rc != TWS_OK) return rc;
+            }
+        }
         s.steps = 0;
     }
     return TWS_OK;
@@ -452,15 +631,7 @@ int tws_batch_step(tws_batch* b, int32_t n) {
 
 int tws_batch_set_launch(tws_batch* b, int32_t kernel, int32_t k) {
     if (!b) return fail(TWS_ERR_INVALID, "tws_batch_set_launch: NULL handle");
-    if (kernel == TWS_KERNEL_AUTO) {
-        // 128x64 tiles at k = 4 for every grid: the autotuner's choice for the
-        // batched mix and for each size class of it.  A class on the
-        // streaming kernel for the grids above 2176^2 (as choose_launch steps
-        // them alone) lowered the served mix's capacity from 3.78e11 to
-        // 3.35-3.38e11 cell-updates/s, run beside the tile chains or after them.
-        kernel = TWS_KERNEL_TILE_128x64;
-        k = 4;
-    }
+    // TWS_KERNEL_AUTO: one class that picks its launch per run (resolve_class)
     const int64_t max_cells = INT64_MAX;
     return tws_batch_set_launch_classes(b, 1, &max_cells, &kernel, &k);
 }
@@ -482,9 +653,9 @@ int tws_batch_set_launch_classes(tws_batch* b, int32_t n, const int64_t* max_cel
     for (int c = 0; c < n; ++c) {
         TileShape sh{};
         int ox = 0;
-        const bool ok = kernels[c] == TWS_KERNEL_STREAM_128
-                            ? ks[c] >= 2 && stream_shape(ks[c], &ox)
-                            : tile_shape(kernels[c], ks[c], &sh);
+        const bool ok = kernels[c] == TWS_KERNEL_AUTO ||
+                        (kernels[c] == TWS_KERNEL_STREAM_128 ? ks[c] >= 2 && stream_shape(ks[c], &ox)
+                                                             : tile_shape(kernels[c], ks[c], &sh));
         if (!ok)
             return fail(TWS_ERR_INVALID, "tws_batch_set_launch_classes: class %d: kernel %d / k %d "
                         "not available for batches", c, kernels[c], ks[c]);
@@ -506,7 +677,8 @@ int tws_batch_set_launch_classes(tws_batch* b, int32_t n, const int64_t* max_cel
     std::vector<LaunchClass> cl;
     try {
         for (int c = 0; c < n; ++c)
-            cl.push_back(LaunchClass{c + 1 < n ? max_cells[c] : INT64_MAX, kernels[c], ks[c]});
+            cl.push_back(LaunchClass{c + 1 < n ? max_cells[c] : INT64_MAX, kernels[c],
+                                     kernels[c] == TWS_KERNEL_AUTO ? 0 : ks[c]});
     } catch (...) {
         return fail(TWS_ERR_NOMEM, "tws_batch_set_launch_classes: host allocation failed");
     }

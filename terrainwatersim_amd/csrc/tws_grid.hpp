@@ -49,6 +49,10 @@ int grid_clear(GridState& g, hipStream_t s);
 // or the two-pass kernels; optional events around each launch.
 int step_grid(GridState& gs, int kernel, int k, int32_t n, hipStream_t stream, hipEvent_t* ev,
               int ev_pairs, int* launches);
+// Whether the streaming kernel can step this grid: closed walls, and each
+// plane smaller than 2 GiB (its stores address a plane with 31-bit offsets
+// and use offset 2^31 for lanes that must not write).
+bool stream_usable(const GridState& gs);
 int grid_upload(GridState& g, const float* terrain, const float* depth, const float* flux4,
                 hipStream_t s);
 int grid_download(GridState& g, float* depth, float* flux4, float* velocity2, hipStream_t s);

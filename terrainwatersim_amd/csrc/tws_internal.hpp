@@ -59,6 +59,9 @@ struct GridDesc {
     // TWS_BOUNDARY_REFERENCE_DISPATCH mode, where the reference's truncating
     // dispatch shapes leave the tail untouched.
     int32_t upd_w, upd_h, brush_w, brush_h;
+    // Streaming kernel in a batch: output rows per segment (tiles_x strips by
+    // tiles_y segments, tasks numbered from tile_begin).
+    int32_t seg_rows;
     unsigned int* status;    // bit 0: non-finite value seen
 };
 
@@ -139,6 +142,20 @@ struct StreamPlan {
 };
 // Output columns per strip at k steps per launch; false if k is not built.
 bool stream_shape(int k, int* ox);
+// A grid's tasks in a batched launch of the streaming kernel: strips of
+// stream_shape() output columns by segments of seg_rows rows, seg_rows the
+// largest even split of H into segments of at most `cap` rows.
+bool stream_batch_plan(int W, int H, int k, int cap, int* strips, int* segs, int* seg_rows);
+// Segment cap (rows) for a batched launch of the streaming kernel over the
+// grids shapes[2i] x shapes[2i+1], i < n, in task order, on a device of `cus`
+// compute units: the cap whose tasks, dealt to the XCDs and run on their
+// resident work-group slots in order, finish soonest.  May throw bad_alloc.
+int stream_batch_cap(const int* shapes, int n, int k, int cus);
+// Batched launch launch_idx of the streaming kernel (closed walls; `mode`
+// kModePlain or kModeRain, the highest of the table's grids): hdr_dev =
+// {grids, tasks, offset}; `blocks` is at least the work groups the tasks need.
+hipError_t launch_stream_batch(int k, int mode, const GridDesc* grids_dev, const int* hdr_dev,
+                               int blocks, int launch_idx, hipStream_t s);
 // Segments of a W x H grid on a device of `cus` compute units: as many as
 // fill its wavefront slots once.
 StreamPlan stream_plan(int W, int H, int k, int cus);

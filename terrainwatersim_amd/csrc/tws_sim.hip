@@ -35,10 +35,7 @@ struct tws_sim {
 
 namespace tws {
 
-// Whether the streaming kernel can step this grid: closed walls, and each
-// plane smaller than 2 GiB (its stores address a plane with 31-bit offsets
-// and use offset 2^31 for lanes that must not write).
-static bool stream_usable(const GridState& gs) {
+bool stream_usable(const GridState& gs) {
     return gs.params.boundary == TWS_BOUNDARY_CLOSED && gs.pitch * gs.H < ((int64_t)1 << 29);
 }
 

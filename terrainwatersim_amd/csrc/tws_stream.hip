@@ -30,6 +30,9 @@
 // to the FP32 restatement for any segment length and steps per launch.
 
 #include <algorithm>
+#include <cstdint>
+#include <functional>
+#include <vector>
 #include <type_traits>
 
 #include "tws.h"
@@ -159,7 +162,11 @@ __device__ __forceinline__ void f_fix(const GridDesc& g, int y, int x0, bool inn
 
 // k_stream: grid of ceil(tasks / kStreamWaves) work groups; task = segment *
 // strips + strip, segments of seg_rows output rows.  Closed walls only
-// (MODE kModePlain or kModeRain).
+// (MODE kModePlain or kModeRain).  Single grid (BATCH false): g1 (by value)
+// is the grid.  Batch: hdr = {grids, tasks, offset} and grids[offset..]
+// (ordered by tile_begin, the grid's first task; tiles_x strips by tiles_y
+// segments of seg_rows rows) live in device memory; launch launch_idx
+// advances the grids with at least (launch_idx + 1) KH steps by KH steps.
 //
 // Registers, per lane and column: level j >= 1 keeps the flux of its rows
 // a, a-1, a-2 in a 3-slot ring FS[j] (the new row a overwrites a-3) and the
@@ -169,9 +176,10 @@ __device__ __forceinline__ void f_fix(const GridDesc& g, int y, int x0, bool inn
 // rows come through PB (b and d, two rows ahead), BR (the terrain of the last
 // two input rows, which level 1 reads) and PF (flux, the row level 1 reads
 // plus one ahead).
-template <int KH, int MODE>
+template <int KH, int MODE, bool BATCH>
 __global__ void __launch_bounds__(kStreamWaves * kWave, StreamGeom<KH>::WAVES_PER_SIMD)
-k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
+k_stream(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict__ hdr,
+         int launch_idx, int strips1, int segs1, int seg_rows1, int parity1) {
     constexpr bool RAIN = MODE == kModeRain;
     using G = StreamGeom<KH>;
     constexpr int kRing = G::RING;
@@ -182,14 +190,56 @@ k_stream(GridDesc g, int strips, int segs, int seg_rows, int parity) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // XCD-aware order: work groups are dealt to the 8 XCDs round robin; each
     // XCD gets a contiguous run of tasks, so neighbouring strips (which
-    // re-read each other's halo columns) meet in one L2.
-    const int nb = (int)gridDim.x, bid = blockIdx.x;
+    // re-read each other's halo columns) meet in one L2.  The runs cover the
+    // work groups that hold tasks, so the spare ones of a batch launch (its
+    // block count is rounded up) fall evenly on the XCDs.
+    const int ntasks = BATCH ? hdr[1] : strips1 * segs1;
+    const int nb = min((int)gridDim.x, (ntasks + kStreamWaves - 1) / kStreamWaves);
+    const int bid = blockIdx.x;
     const int xcd = bid & 7, idx = bid >> 3;
     const int q = nb >> 3, rmd = nb & 7;
+    if (idx >= q + (xcd < rmd ? 1 : 0)) return;  // whole work group: no barrier follows
     const int blk = xcd * q + (xcd < rmd ? xcd : rmd) + idx;
     const int task = blk * kStreamWaves + wave;
-    if (task >= strips * segs) return;  // whole wavefront: no barrier follows
-    const int seg = task / strips, strip = task - seg * strips;
+    if (task >= ntasks) return;  // whole wavefront: no barrier follows
+    GridDesc g;
+    int strips, seg_rows, parity, local;
+    if constexpr (!BATCH) {
+        g = g1;
+        strips = strips1;
+        seg_rows = seg_rows1;
+        parity = parity1;
+        local = task;
+    } else {
+        const GridDesc* table = grids + hdr[2];
+        int lo = 0, hi = hdr[0] - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (table[mid].tile_begin <= task) lo = mid; else hi = mid - 1;
+        }
+        g = table[__builtin_amdgcn_readfirstlane(lo)];  // wave-uniform: scalar loads
+        if (g.steps < (launch_idx + 1) * KH) return;  // its remaining steps run elsewhere
+        strips = g.tiles_x;
+        seg_rows = g.seg_rows;
+        local = task - g.tile_begin;
+        // put this launch's input set at index 0, so that every index into
+        // the copy's ping-pong arrays is a constant and the copy can live in
+        // registers (a runtime index would keep it in private memory)
+        const bool odd = ((g.parity0 + launch_idx) & 1) != 0;
+        float* const d_in = odd ? g.d[1] : g.d[0];
+        float* const d_out = odd ? g.d[0] : g.d[1];
+        g.d[0] = d_in;
+        g.d[1] = d_out;
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) {
+            float* const f_in = odd ? g.f[1][ch] : g.f[0][ch];
+            float* const f_out = odd ? g.f[0][ch] : g.f[1][ch];
+            g.f[0][ch] = f_in;
+            g.f[1][ch] = f_out;
+        }
+        parity = 0;
+    }
+    const int seg = local / strips, strip = local - seg * strips;
     const int Y0 = seg * seg_rows;
     const int Y1 = min(g.H, Y0 + seg_rows);
     if (Y0 >= Y1) return;
@@ -539,20 +589,29 @@ StreamPlan stream_plan(int W, int H, int k, int cus) {
     return p;
 }
 
-template <int MODE>
-static hipError_t launch_stream_m(int k, const GridDesc& g, const StreamPlan& p, int parity,
-                                  hipStream_t s) {
+// One launch of k levels: a single grid (grids null; g1, the plan and the
+// parity by value) or a batch (the table, launch launch_idx).
+template <int MODE, bool BATCH>
+static hipError_t launch_stream_m(int k, const GridDesc& g1, const GridDesc* grids,
+                                  const int* hdr, int launch_idx, const StreamPlan& p,
+                                  int parity, hipStream_t s) {
     const dim3 grid(p.blocks), block(kStreamWaves * kWave);
+#define TWS_STREAM_CASE(KK)                                                                   \
+    case KK:                                                                                  \
+        hipLaunchKernelGGL((k_stream<KK, MODE, BATCH>), grid, block, 0, s, g1, grids, hdr,    \
+                           launch_idx, p.strips, p.segs, p.seg_rows, parity);                 \
+        break;
     switch (k) {
-        case 1: hipLaunchKernelGGL((k_stream<1, MODE>), grid, block, 0, s, g, p.strips, p.segs, p.seg_rows, parity); break;
-        case 2: hipLaunchKernelGGL((k_stream<2, MODE>), grid, block, 0, s, g, p.strips, p.segs, p.seg_rows, parity); break;
-        case 3: hipLaunchKernelGGL((k_stream<3, MODE>), grid, block, 0, s, g, p.strips, p.segs, p.seg_rows, parity); break;
-        case 4: hipLaunchKernelGGL((k_stream<4, MODE>), grid, block, 0, s, g, p.strips, p.segs, p.seg_rows, parity); break;
-        case 5: hipLaunchKernelGGL((k_stream<5, MODE>), grid, block, 0, s, g, p.strips, p.segs, p.seg_rows, parity); break;
-        case 6: hipLaunchKernelGGL((k_stream<6, MODE>), grid, block, 0, s, g, p.strips, p.segs, p.seg_rows, parity); break;
-        case 8: hipLaunchKernelGGL((k_stream<8, MODE>), grid, block, 0, s, g, p.strips, p.segs, p.seg_rows, parity); break;
+        TWS_STREAM_CASE(1)
+        TWS_STREAM_CASE(2)
+        TWS_STREAM_CASE(3)
+        TWS_STREAM_CASE(4)
+        TWS_STREAM_CASE(5)
+        TWS_STREAM_CASE(6)
+        TWS_STREAM_CASE(8)
         default: return hipErrorInvalidValue;
     }
+#undef TWS_STREAM_CASE
     return hipGetLastError();
 }
 
@@ -560,8 +619,116 @@ hipError_t launch_stream_single(int k, const GridDesc& g, const StreamPlan& p, i
                                 hipStream_t s) {
     if (p.blocks <= 0) return hipErrorInvalidValue;
     if (g.boundary != TWS_BOUNDARY_CLOSED) return hipErrorInvalidValue;
-    return g.rain_on ? launch_stream_m<kModeRain>(k, g, p, parity, s)
-                     : launch_stream_m<kModePlain>(k, g, p, parity, s);
+    return g.rain_on ? launch_stream_m<kModeRain, false>(k, g, nullptr, nullptr, 0, p, parity, s)
+                     : launch_stream_m<kModePlain, false>(k, g, nullptr, nullptr, 0, p, parity, s);
+}
+
+// Rows per segment of an H-row grid in a batch: the largest even split into
+// segments of at most `cap` rows, but none shorter than 4 k rows when the
+// grid has more than one (as stream_plan keeps them).
+static int batch_seg_rows(int H, int k, int cap) {
+    const int n = std::min((H + cap - 1) / cap, std::max(1, H / (4 * k)));
+    return (H + n - 1) / n;
+}
+
+bool stream_batch_plan(int W, int H, int k, int cap, int* strips, int* segs, int* seg_rows) {
+    int ox = 0;
+    if (!stream_shape(k, &ox) || W <= 0 || H <= 0 || cap <= 0) return false;
+    *seg_rows = batch_seg_rows(H, k, cap);
+    *segs = (H + *seg_rows - 1) / *seg_rows;
+    *strips = (W + ox - 1) / ox;
+    return true;
+}
+
+// Caps tried for a batch, in rows: kStreamSegRows down to 4 k.
+static constexpr int kBatchCapStep = 8;
+
+int stream_batch_cap(const int* shapes, int n, int k, int cus) {
+    int ox = 0, per_simd = 0;
+    if (!stream_geom(k, &ox, &per_simd) || !shapes || n <= 0 || cus < 8) return kStreamSegRows;
+    // A batch launch is a few rounds of tasks of unequal length (a grid
+    // shorter than the cap is one segment of its own height), and a last
+    // round that is nearly empty costs nearly a whole round: MI355X, 256
+    // grids of 128^2..2047^2 with rain, k = 6, cell-updates/s by cap: 96 rows
+    // 3.97e11, 128 rows 4.01e11, 192 rows 4.22e11, 320 rows 3.72e11 (3.15
+    // rounds).  So each cap is priced by replaying the dispatch: the work
+    // groups (4 consecutive tasks; one lasts as long as its longest task plus
+    // a ramp of 4 k rows) are dealt to the 8 XCDs in contiguous runs, as the
+    // kernel maps them, and each XCD starts its run in order on its resident
+    // slots.  The cap whose slowest XCD ends first wins; ties keep the
+    // longer segments.  Shorter caps only add tasks, so the search stops
+    // once the tasks fill 8 rounds, past which the last round's idle slots
+    // cost less than the ramps that more segments add.
+    const int slots = std::max(1, cus / 8 * per_simd);  // work groups resident per XCD
+    const int64_t max_tasks = (int64_t)8 * 8 * slots * kStreamWaves;
+    struct Run {
+        int64_t count;
+        int rows;
+    };
+    std::vector<Run> runs;  // the tasks in order: runs of equal segment rows
+    std::vector<int64_t> heap;
+    int best = kStreamSegRows;
+    int64_t best_span = INT64_MAX;
+    for (int cap = kStreamSegRows; cap >= 4 * k; cap -= kBatchCapStep) {
+        runs.clear();
+        int64_t nt = 0;
+        for (int i = 0; i < n; ++i) {
+            const int W = shapes[2 * i], H = shapes[2 * i + 1];
+            if (W <= 0 || H <= 0) continue;
+            const int sr = batch_seg_rows(H, k, cap), segs = (H + sr - 1) / sr;
+            const int strips = (W + ox - 1) / ox;
+            if (segs > 1) runs.push_back(Run{(int64_t)strips * (segs - 1), sr});
+            runs.push_back(Run{strips, H - (segs - 1) * sr});
+            nt += (int64_t)strips * segs;
+        }
+        if (nt == 0) break;
+        if (nt > max_tasks && cap < kStreamSegRows) break;
+        const int64_t nb = (nt + kStreamWaves - 1) / kStreamWaves;
+        const int64_t q = nb / 8, rmd = nb % 8;
+        size_t ri = 0;
+        int64_t left = runs[0].count;
+        int64_t span = 0;
+        for (int x = 0; x < 8; ++x) {
+            const int64_t cnt = q + (x < rmd ? 1 : 0);
+            heap.assign((size_t)std::min<int64_t>(slots, std::max<int64_t>(cnt, 1)), 0);
+            for (int64_t j = 0; j < cnt; ++j) {
+                // the work group's tasks: the next kStreamWaves of the runs
+                int d = 0;
+                for (int take = kStreamWaves; take > 0 && ri < runs.size();) {
+                    const int64_t t = std::min<int64_t>(take, left);
+                    d = std::max(d, runs[ri].rows);
+                    take -= (int)t;
+                    left -= t;
+                    if (left == 0 && ++ri < runs.size()) left = runs[ri].count;
+                }
+                std::pop_heap(heap.begin(), heap.end(), std::greater<int64_t>());
+                heap.back() += d + 4 * k;
+                std::push_heap(heap.begin(), heap.end(), std::greater<int64_t>());
+            }
+            span = std::max(span, *std::max_element(heap.begin(), heap.end()));
+        }
+        if (span < best_span) {
+            best_span = span;
+            best = cap;
+        }
+    }
+    return best;
+}
+
+hipError_t launch_stream_batch(int k, int mode, const GridDesc* grids_dev, const int* hdr_dev,
+                               int blocks, int launch_idx, hipStream_t s) {
+    if (blocks <= 0) return hipErrorInvalidValue;
+    StreamPlan p{};
+    p.blocks = blocks;
+    const GridDesc g1{};
+    switch (mode) {
+        case kModePlain:
+            return launch_stream_m<kModePlain, true>(k, g1, grids_dev, hdr_dev, launch_idx, p, 0, s);
+        case kModeRain:
+            return launch_stream_m<kModeRain, true>(k, g1, grids_dev, hdr_dev, launch_idx, p, 0, s);
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace tws

@@ -8,8 +8,9 @@
 // finish sooner), 128 x 64 tiles beyond (two work groups per CU so one loads
 // while the other computes), 5 steps per launch up to 4096^2 and 4 above;
 // grids above 2176^2 in closed walls take the streaming kernel at k = 6.
-// tws_autotune() times every candidate on scratch grids of the requested
-// shapes and returns the winner; only tws_tune_record() enters a launch into
+// tws_autotune() times every candidate (the tile variants and the streaming
+// kernel, each at k = 2..6 and 8) on scratch grids of the requested shapes
+// advanced as one batch and returns the winner; only tws_tune_record() enters a launch into
 // the per-size-class table (log2 of the cell count) that choose_launch()
 // consults first, so tuning a batch never changes what a later plain grid
 // of the process runs.
@@ -198,9 +199,10 @@ int tws_autotune(const int32_t* widths, const int32_t* heights, int32_t n, int32
         if (rc == TWS_OK)
             rc = tws_batch_set_rain(b, id, 0.5f * widths[i], 0.5f * heights[i], 1e-3f, 256.0f);
     }
-    static const int kernels[] = {TWS_KERNEL_TILE_128x64, TWS_KERNEL_TILE_128x128,
-                                  TWS_KERNEL_TILE_256x64, TWS_KERNEL_TILE_64x64,
-                                  TWS_KERNEL_TILE_64x64_16W, TWS_KERNEL_TILE_64x48_16W};
+    static const int kernels[] = {TWS_KERNEL_TILE_128x64,    TWS_KERNEL_TILE_128x128,
+                                  TWS_KERNEL_TILE_256x64,    TWS_KERNEL_TILE_64x64,
+                                  TWS_KERNEL_TILE_64x64_16W, TWS_KERNEL_TILE_64x48_16W,
+                                  TWS_KERNEL_STREAM_128};
     static const int ks[] = {2, 3, 4, 5, 6, 8};
     tws_tune_entry top{0, 0, 0.0};
     int m = 0;
@@ -235,7 +237,8 @@ int tws_tune_record(int32_t width, int32_t height, int32_t kernel, int32_t k) {
     if (width <= 0 || height <= 0)
         return fail(TWS_ERR_INVALID, "tws_tune_record: invalid size %d x %d", width, height);
     TileShape sh{};
-    if (!tile_shape(kernel, k, &sh))
+    int ox = 0;
+    if (!(kernel == TWS_KERNEL_STREAM_128 ? k >= 2 && stream_shape(k, &ox) : tile_shape(kernel, k, &sh)))
         return fail(TWS_ERR_INVALID, "tws_tune_record: unknown kernel %d or k %d", kernel, k);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     try {

@@ -94,7 +94,9 @@ def test_tune_record_validates_and_forget_clears(tws):
     assert lib.tws_tune_record(64, 64, 99, 4) == -1
     assert lib.tws_tune_record(64, 64, tws.KERNEL_TILE_128x64, 7) == -1
     assert b"tws_tune_record" in lib.tws_last_error()
+    assert lib.tws_tune_record(64, 64, tws.KERNEL_STREAM_128, 7) == -1
     assert lib.tws_tune_record(64, 64, tws.KERNEL_TILE_128x64, 4) == 0
+    assert lib.tws_tune_record(4096, 4096, tws.KERNEL_STREAM_128, 6) == 0
     assert lib.tws_tune_forget() == 0
 
 

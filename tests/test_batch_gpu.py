@@ -190,9 +190,10 @@ def test_autotune_records_only_on_request(gpu):
         builtin = s.get_launch()
     try:
         best, trials = gpu.autotune(shapes, steps=16)
-        assert len(trials) == len(gpu.TILE_KERNELS) * 6
+        kernels = list(gpu.TILE_KERNELS) + [gpu.KERNEL_STREAM_128]
+        assert len(trials) == len(kernels) * 6
         assert {(t["kernel"], t["k"]) for t in trials} == {
-            (kern, k) for kern in gpu.TILE_KERNELS for k in (2, 3, 4, 5, 6, 8)}
+            (kern, k) for kern in kernels for k in (2, 3, 4, 5, 6, 8)}
         assert all(t["cell_updates_per_s"] > 0 for t in trials)
         top = max(trials, key=lambda t: t["cell_updates_per_s"])
         assert (best["kernel"], best["k"]) == (top["kernel"], top["k"])
@@ -279,12 +280,12 @@ def test_launch_classes_bitwise(gpu, oracle, graphs):
 
 def test_launch_classes_rejects(gpu):
     """Invalid class lists leave the classes as they were; the default (and
-    TWS_KERNEL_AUTO) is one class, 128x64 tiles at k = 4."""
+    TWS_KERNEL_AUTO) is one class of kernel AUTO, whose k reads 0."""
     with gpu.Batch(4, 64 << 20) as batch:
-        T, S = gpu.KERNEL_TILE_128x64, gpu.KERNEL_STREAM_128
-        default = [(2**63 - 1, T, 4)]
+        T, S, A = gpu.KERNEL_TILE_128x64, gpu.KERNEL_STREAM_128, gpu.KERNEL_AUTO
+        default = [(2**63 - 1, A, 0)]
         assert batch.get_launch_classes() == default
-        assert batch.get_launch() == (T, 4)
+        assert batch.get_launch() == (A, 0)
         for bad in ([], [(0, T, 4)] * 5, [(500, T, 4), (400, T, 4), (0, T, 4)],
                     [(0, T, 4), (0, T, 4)], [(0, S, 7)], [(0, S, 1)], [(0, T, 7)],
                     [(0, gpu.KERNEL_TWOPASS, 1)]):
@@ -295,6 +296,8 @@ def test_launch_classes_rejects(gpu):
         assert batch.get_launch_classes() == [(2**63 - 1, gpu.KERNEL_TILE_64x64, 5)]
         batch.set_launch(gpu.KERNEL_AUTO, 0)
         assert batch.get_launch_classes() == default
+        batch.set_launch_classes([(1000, A, 7), (0, T, 4)])
+        assert batch.get_launch_classes() == [(1000, A, 0), (2**63 - 1, T, 4)]
 
 
 STREAM_CLASS_SHAPES = [(1, 1), (90, 110), (250, 240), (520, 700), (640, 300), (301, 1033)]
@@ -303,10 +306,10 @@ STREAM_CLASS_SHAPES = [(1, 1), (90, 110), (250, 240), (520, 700), (640, 300), (3
 @pytest.mark.parametrize("graphs", [True, False])
 def test_launch_classes_with_stream_bitwise(gpu, oracle, graphs):
     """A batch whose largest grids fall into a class on the streaming kernel
-    (stepped with direct launches after the replayed tile chains), a grid on
-    the reference's edge in that class (it steps on the tile table's launch),
-    per-grid step counts: three runs, each grid after each run bit-identical
-    to the FP32 restatement."""
+    (its batched chain beside the tile chain in one graph, the steps beyond
+    full launches on direct launches), a grid on the reference's edge in that
+    class (it steps on the tile table's launch), per-grid step counts: three
+    runs, each grid after each run bit-identical to the FP32 restatement."""
     classes = [(300 * 300, gpu.KERNEL_TILE_64x64_16W, 3), (0, gpu.KERNEL_STREAM_128, 5)]
     grids = []
     for i, (H, W) in enumerate(STREAM_CLASS_SHAPES):
@@ -335,3 +338,85 @@ def test_launch_classes_with_stream_bitwise(gpu, oracle, graphs):
                 assert np.array_equal(out["depth"], d32), (run, STREAM_CLASS_SHAPES[j])
                 assert np.array_equal(out["flux"], f32), (run, STREAM_CLASS_SHAPES[j])
                 assert batch.check_finite(gid)
+
+
+CHAIN_SHAPES = [(1, 1), (2, 3), (40, 33), (300, 200), (700, 130), (129, 900), (3000, 2), (77, 401)]
+
+
+# Steps per grid of CHAIN_SHAPES by k: every remainder a tail launch takes
+# (1..min(k-1, 6)), whole launches, counts below k, and for k = 8 the
+# remainder 7 that steps on direct launches.
+CHAIN_STEPS = {6: [6, 3, 12, 17, 7, 25, 6, 5], 8: [8, 7, 15, 17, 9, 22, 6, 3],
+               4: [4, 3, 9, 17, 6, 25, 2, 5]}
+
+
+@pytest.mark.parametrize("k,graphs", [(6, True), (6, False), (8, True), (4, True)])
+def test_stream_chain_bitwise(gpu, oracle, k, graphs):
+    """Every grid on the streaming kernel's batched chain (one table of all
+    their strip segments, full launches of k steps, then one tail launch per
+    remainder): degenerate and tall or wide shapes, rain and no rain mixed in
+    one launch, a grid in zero-edge mode (which the streaming kernel leaves to
+    the tile table's launch), step counts below k, at k and not a multiple of
+    k; three runs, the last with the step counts rotated (other tails), each
+    grid after each run bit-identical to the FP32 restatement."""
+    grids = []
+    for i, (H, W) in enumerate(CHAIN_SHAPES):
+        b, d, f = state(700 + i, H, W)
+        bd = gpu.BOUNDARY_ZERO if i == 3 else gpu.BOUNDARY_CLOSED
+        p = gpu.params_physical(0.02 + 0.001 * i, 9.81, 1.0, 1.0, 0.995, bd)
+        rain = (W / 3.0, H / 2.0, 5e-3, 100.0) if i % 3 == 0 else None
+        grids.append((b, d, f, p, rain))
+    oracle.set_threads(16)
+    want = [(d.copy(), f.copy()) for _, d, f, _, _ in grids]
+    with gpu.Batch(len(grids), 1 * GB) as batch:
+        batch.use_graphs(graphs)
+        batch.set_launch_classes([(0, gpu.KERNEL_STREAM_128, k)])
+        ids = load_batch(gpu, batch, grids)
+        for run in range(3):
+            steps = CHAIN_STEPS[k] if run < 2 else CHAIN_STEPS[k][1:] + CHAIN_STEPS[k][:1]
+            for gid, n in zip(ids, steps):
+                batch.set_steps(gid, n)
+            batch.run()
+            for j, ((b, _, _, p, rain), gid, n) in enumerate(zip(grids, ids, steps)):
+                F, K, Cc, bd, _ = p.as_tuple()
+                d32, f32, _ = oracle.step(b, want[j][0], want[j][1], F, K, Cc, n=n, boundary=bd,
+                                          rain=rain)
+                want[j] = (d32, f32)
+                out = batch.download(gid, flux=True)
+                assert np.array_equal(out["depth"], d32), (run, CHAIN_SHAPES[j])
+                assert np.array_equal(out["flux"], f32), (run, CHAIN_SHAPES[j])
+
+
+@pytest.mark.parametrize("big", [True, False])
+def test_auto_class_matches_tiles(gpu, big):
+    """The default AUTO class picks per run: the streaming chain (full
+    launches, tails, the fallback tile table for a zero-edge grid) when the
+    grids it can step hold more than 2^23 cells (big), the tile launch
+    otherwise.  Either way every grid equals the same batch on 64x64 tiles
+    at k = 5 after each of three runs of mixed step counts."""
+    shapes = [(257, 130), (300, 200), (64, 64), (1, 700)] + ([(2900, 2900)] if big else [])
+    grids = []
+    for i, (H, W) in enumerate(shapes):
+        b, d, f = state(900 + i, H, W)
+        bd = gpu.BOUNDARY_ZERO if i == 0 else gpu.BOUNDARY_CLOSED
+        p = gpu.params_physical(0.02 + 0.001 * i, 9.81, 1.0, 1.0, 0.995, bd)
+        rain = (W / 3.0, H / 2.0, 5e-3, 100.0) if i % 2 == 0 else None
+        grids.append((b, d, f, p, rain))
+    runs = [[13] * len(shapes), [6, 100, 1, 0, 7][:len(shapes)], [25, 3, 12, 5, 31][:len(shapes)]]
+    out = {}
+    for name, launch in (("auto", None), ("tiles", (gpu.KERNEL_TILE_64x64, 5))):
+        with gpu.Batch(len(grids), 1 * GB) as batch:
+            if launch:
+                batch.set_launch(*launch)
+            ids = load_batch(gpu, batch, grids)
+            res = []
+            for steps in runs:
+                for gid, n in zip(ids, steps):
+                    batch.set_steps(gid, n)
+                batch.run()
+                res.append([batch.download(gid, flux=True) for gid in ids])
+            out[name] = res
+    for r, (a, t) in enumerate(zip(out["auto"], out["tiles"])):
+        for j in range(len(shapes)):
+            assert np.array_equal(a[j]["depth"], t[j]["depth"]), (r, shapes[j])
+            assert np.array_equal(a[j]["flux"], t[j]["flux"]), (r, shapes[j])

@@ -5,11 +5,12 @@ small    256^2 fBm terrain (seed 1), dry start, rain disc r = 16 (R^2 = 256)
          1000 steps: the latency/launch-bound case (state fits in L2/MALL).
 batched  256 independent terrains, sides log-uniform 128..2048 (seed 3),
          per-grid random rain discs (r in [8, 32], 1e-3 m/step), 100 steps per
-         captured launch sequence.  Three size classes (sides <= 512, <= 1024,
-         larger), each on the (variant, k) the autotuner picks for its grids,
-         as concurrent launch chains; beside it the whole mix on the one
-         launch the autotuner picks for all of it.  Compared with the same
-         grids stepped one at a time.
+         captured launch sequence, in three layouts: the batch's default
+         launch (TWS_KERNEL_AUTO); the one launch the autotuner picks for the
+         whole mix; three size classes (sides <= 512, <= 1024, larger), each
+         on the (kernel, k) the autotuner picks for its grids, as concurrent
+         launch chains.  The fastest is the batched rate.  Compared with the
+         same grids stepped one at a time.
 served   Poisson stream of solves, sides log-uniform 256..4096, steps uniform
          100..5000, dam-break (20 m column over a centred square of side/8)
          or rain (r = 16 at the centre, 1e-3 m/step) 50/50, arrival rate 70%
@@ -151,17 +152,28 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
                 batch.upload(gid, b, np.zeros((s, s), F32))
                 batch.set_rain(gid, *rain)
 
-        load()
-        batch.set_launch(best["kernel"], best["k"])
-        t1 = _time_runs(batch, steps, repeats)
-        out["batched_one_launch_cell_updates_per_s"] = cells * steps / t1
-        load()
-        batch.set_launch_classes(classes)
-        t = _time_runs(batch, steps, repeats)
-        out["batched_cell_updates_per_s"] = cells * steps / t
-        out["batched_ms_per_run"] = t * 1e3
+        # every layout runs the same steps from the same state, so their
+        # results must agree bit for bit
+        times, layout_checks = {}, {}
+        for layout in ("default", "one_launch", "classes"):
+            load()
+            if layout == "default":
+                batch.set_launch(tws.KERNEL_AUTO, 0)
+            elif layout == "one_launch":
+                batch.set_launch(best["kernel"], best["k"])
+            else:
+                batch.set_launch_classes(classes)
+            times[layout] = _time_runs(batch, steps, repeats)
+            out[f"batched_{layout}_cell_updates_per_s"] = cells * steps / times[layout]
+            layout_checks[layout] = [batch.download(g, flux=False)["depth"] for g in ids[:4]]
         out["graph"] = batch.stats()
-        checks = [batch.download(g, flux=False)["depth"] for g in ids[:4]]
+        fastest = min(times, key=times.get)
+        out["batched_layout"] = fastest
+        out["batched_cell_updates_per_s"] = cells * steps / times[fastest]
+        out["batched_ms_per_run"] = times[fastest] * 1e3
+        checks = layout_checks[fastest]
+        out["layouts_equal"] = all(
+            np.array_equal(a, b) for lc in layout_checks.values() for a, b in zip(lc, checks))
     # The same grids one at a time, each stepped and synchronised alone: first
     # on its own default launch (what a caller of the plain path gets; the
     # tuner's table is cleared so the batch's result does not leak into it),
@@ -177,12 +189,13 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
         if not forced:
             out["plain_launches"] = launches
     out["batched_over_plain"] = out["batched_cell_updates_per_s"] / out["plain_cell_updates_per_s"]
-    out["classes_over_one_launch"] = (out["batched_cell_updates_per_s"]
+    out["classes_over_one_launch"] = (out["batched_classes_cell_updates_per_s"]
                                       / out["batched_one_launch_cell_updates_per_s"])
     out["batched_over_plain_forced"] = (out["batched_cell_updates_per_s"]
                                         / out["plain_forced_cell_updates_per_s"])
     out["batched_equals_plain"] = bool(out["plain_equals_batched"]
-                                       and out["plain_forced_equals_batched"])
+                                       and out["plain_forced_equals_batched"]
+                                       and out["layouts_equal"])
     return out
 
 

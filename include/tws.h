@@ -353,7 +353,8 @@ int tws_batch_step(tws_batch* batch, int32_t n);
  * with one class.  TWS_KERNEL_AUTO (the default; reported as kernel
  * TWS_KERNEL_AUTO, k 0) picks per run: TWS_KERNEL_STREAM_128 at k = 6 when the
  * grids with pending steps that the streaming kernel can step hold more than
- * 2^23 cells (2896^2) together, 128x64 tiles at k = 4 otherwise.
+ * 2^23 cells (2896^2) together; otherwise the tile launch TWS_KERNEL_AUTO
+ * gives one grid of all their cells (on other edges).
  * tws_batch_get_launch reports the class of the largest grids. */
 int tws_batch_set_launch(tws_batch* batch, int32_t kernel, int32_t steps_per_launch);
 int tws_batch_get_launch(const tws_batch* batch, int32_t* kernel, int32_t* steps_per_launch);

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -423,24 +424,32 @@ static bool in_stream_tail(const Slot& s, int k) {
 
 // The launch class c runs this run.  An AUTO class takes the streaming kernel
 // at k = 6 when its grids with pending steps that the streaming kernel can
-// step hold more than 2^23 cells (2896^2) together, and the fallback tile
-// launch otherwise.  MI355X, ms per 100 steps with a rain disc, streaming
-// k = 6 vs 128x64 tiles at k = 4: 20 x 512^2 2.22 vs 2.01, 1 x 2304^2 2.16
-// vs 1.81; at 2^23 cells 32 x 512^2 2.92 vs 2.87, 8 x 1024^2 2.77 vs 2.86,
-// 2 x 2048^2 2.89 vs 2.97; 1 x 3072^2 2.80 vs 3.06, 1 x 4096^2 4.51 vs 4.87;
-// 256 grids of 128^2..2047^2 4.21e11 vs 3.84e11 cell-updates/s, and the
-// saturated served mix 4.25e11 vs 3.79e11.
+// step hold more than 2^23 cells (2896^2) together.  MI355X, ms per 100 steps
+// with a rain disc, streaming k = 6 vs 128x64 tiles at k = 4: 20 x 512^2
+// 2.22 vs 2.01, 1 x 2304^2 2.16 vs 1.81; at 2^23 cells 32 x 512^2 2.92 vs
+// 2.87, 8 x 1024^2 2.77 vs 2.86, 2 x 2048^2 2.89 vs 2.97; 1 x 3072^2 2.80 vs
+// 3.06, 1 x 4096^2 4.51 vs 4.87; 256 grids of 128^2..2047^2 4.21e11 vs
+// 3.84e11 cell-updates/s, and the saturated served mix 4.25e11 vs 3.79e11.
+// Otherwise it takes the tile table's launch for one square grid of all its
+// grids' cells (the rounds of tiles are what the table is tuned on): 1 x
+// 256^2 0.20 ms vs 0.60, 64 x 128^2 0.74 vs 0.97, 16 x 256^2 0.70 vs 0.89.
 static constexpr int64_t kAutoStreamCells = (int64_t)1 << 23;
 static LaunchClass resolve_class(const tws_batch* b, int c) {
     LaunchClass lc = b->classes[c];
     if (lc.kernel != TWS_KERNEL_AUTO) return lc;
-    int64_t cells = 0;
+    int64_t cells = 0, all = 0;
     for (const Slot& s : b->slots)
-        if (s.used && s.steps > 0 && stream_usable(s.g) &&
-            class_of(b, (int64_t)s.g.W * s.g.H) == c)
-            cells += (int64_t)s.g.W * s.g.H;
-    lc.kernel = cells > kAutoStreamCells ? TWS_KERNEL_STREAM_128 : kAltKernel;
-    lc.k = cells > kAutoStreamCells ? 6 : kAltK;
+        if (s.used && s.steps > 0 && class_of(b, (int64_t)s.g.W * s.g.H) == c) {
+            all += (int64_t)s.g.W * s.g.H;
+            if (stream_usable(s.g)) cells += (int64_t)s.g.W * s.g.H;
+        }
+    if (cells > kAutoStreamCells) {
+        lc.kernel = TWS_KERNEL_STREAM_128;
+        lc.k = 6;
+    } else {
+        const int side = (int)std::sqrt((double)std::max<int64_t>(all, 1));
+        tile_launch(side, side, &lc.kernel, &lc.k);
+    }
     return lc;
 }
 

@@ -3,7 +3,7 @@
 //
 // choose_launch() maps a grid size to (kernel variant, steps per launch).
 // The built-in table comes from an MI355X sweep over sizes 128^2..16384^2:
-// 64 x 64 tiles (16 waves up to 512^2, 8 waves up to 2048^2: small grids
+// 64 x 64 tiles (16 waves up to 512^2, 8 waves up to 2816^2: small grids
 // are bound by the latency of one tile's levels, so more and smaller tiles
 // finish sooner), 128 x 64 tiles beyond (two work groups per CU so one loads
 // while the other computes), 5 steps per launch up to 4096^2 and 4 above;
@@ -109,10 +109,16 @@ void tile_launch(int width, int height, int* kernel, int* k) {
     //              vs 8.7e10 (8 waves, k = 6) vs 5.1e10 (128x64, k = 5);
     //              64x48 tiles: 2.93 ms per 1000 steps at best vs 2.38
     //   <= 832^2   bands by tile rounds (band_launch)
-    //   <= 2048^2  64x64 tiles, 8 waves, k = 4: 2048^2 2.98e11 with rain
+    //   <= 2304^2  64x64 tiles, 8 waves, k = 4: 2048^2 2.98e11 with rain
     //              (3.31e11 without) vs 2.67e11 (2.91e11) for 128x64, k = 5;
-    //              1024^2 1.88e11 with rain vs 1.72e11
-    //   <= 4096^2  128x64, k = 5: 3.36e11 with rain (3.59e11 without) vs
+    //              1024^2 1.88e11 with rain vs 1.72e11; 2304^2 zero edge, ms
+    //              per 100 steps without (with) rain: 1.61 (1.62) vs 1.91
+    //              (1.81) for 128x64, k = 5
+    //   <= 2816^2  64x64, k = 5 (zero edge, ms per 100 steps without (with)
+    //              rain): 2560^2 2.09 (2.10) vs 2.31 (2.15) for 128x64, k = 5
+    //              and 2.14 (2.16) at k = 4; 2816^2 2.50 (2.53) vs 2.57 (2.53)
+    //   <= 4096^2  128x64, k = 5: 3072^2 2.93 (2.90) ms vs 2.98 (3.01) for 64x64,
+    //              k = 5; 4096^2 3.36e11 with rain (3.59e11 without) vs
     //              3.08e11 (3.42e11) for 64x64, k = 4
     //   larger     128x64, k = 4: the 16384^2 dam-break 4.30e11 vs 4.10e11 at
     //              k = 5 (on an all-wet 16384^2 sheet k = 5 leads by 4%).
@@ -127,9 +133,12 @@ void tile_launch(int width, int height, int* kernel, int* k) {
         *k = 8;
     } else if (cells <= 832 * 832) {
         band_launch(cells, kernel, k);
-    } else if (cells <= 2048 * 2048) {
+    } else if (cells <= 2304 * 2304) {
         *kernel = TWS_KERNEL_TILE_64x64;
         *k = 4;
+    } else if (cells <= 2816 * 2816) {
+        *kernel = TWS_KERNEL_TILE_64x64;
+        *k = 5;
     } else if (cells <= 4096 * 4096) {
         *kernel = TWS_KERNEL_TILE_128x64;
         *k = 5;

@@ -397,3 +397,40 @@ def large(steps=10000, N=16384, seed=5, log=print):
                 "volume_initial_m3": v0, "volume_final_m3": v1,
                 "volume_rel_drift": abs(v1 - v0) / v0})
     return out
+
+
+def batched_default(n_grids=256, steps=100, runs=3, seed=3, log=print):
+    """The batched mix on the batch's default launch alone: one untimed run
+    (the graph capture), then `runs` timed runs of `steps` steps.  The short
+    form that a profiler wraps (no autotuning, no one-at-a-time legs)."""
+    p = tws.params_physical(**PARAMS)
+    grids = batch_grids(n_grids, seed)
+    cells = int(sum(s * s for s, _, _ in grids))
+    pool = sum(tws.grid_footprint(s, s) for s, _, _ in grids)
+    with tws.Batch(n_grids, pool) as batch:
+        for s, b, rain in grids:
+            gid = batch.add(s, s, p)
+            batch.upload(gid, b, np.zeros((s, s), F32))
+            batch.set_rain(gid, *rain)
+        t = _time_runs(batch, steps, runs)
+        log(f"batched_default: {n_grids} grids, {cells / 1e6:.1f} M cells, {steps} steps per "
+            f"run: {cells * steps / t:.4g} cell-updates/s ({t * 1e3:.2f} ms per run)")
+        return {"grids": n_grids, "cells": cells, "steps_per_run": steps,
+                "cell_updates_per_s": cells * steps / t, "ms_per_run": t * 1e3,
+                "graph": batch.stats()}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="Run one workload of this module and print its "
+                                             "record as JSON.")
+    ap.add_argument("workload", choices=["small", "batched", "batched_default", "large"])
+    ap.add_argument("--runs", type=int, default=3, help="timed runs (batched_default)")
+    a = ap.parse_args()
+    if a.workload == "batched_default":
+        rec = batched_default(runs=a.runs)
+    else:
+        rec = globals()[a.workload]()
+    print(json.dumps(rec, default=float))

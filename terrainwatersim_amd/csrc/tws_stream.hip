@@ -658,7 +658,9 @@ int stream_batch_cap(const int* shapes, int n, int k, int cus) {
     // slots.  The cap whose slowest XCD ends first wins; ties keep the
     // longer segments.  Shorter caps only add tasks, so the search stops
     // once the tasks fill 8 rounds, past which the last round's idle slots
-    // cost less than the ramps that more segments add.
+    // cost less than the ramps that more segments add.  A candidate's replay
+    // stops as soon as some slot ends no earlier than the best cap's slowest
+    // XCD: it cannot win any more.
     const int slots = std::max(1, cus / 8 * per_simd);  // work groups resident per XCD
     const int64_t max_tasks = (int64_t)8 * 8 * slots * kStreamWaves;
     struct Run {
@@ -688,10 +690,10 @@ int stream_batch_cap(const int* shapes, int n, int k, int cus) {
         size_t ri = 0;
         int64_t left = runs[0].count;
         int64_t span = 0;
-        for (int x = 0; x < 8; ++x) {
+        for (int x = 0; x < 8 && span < best_span; ++x) {
             const int64_t cnt = q + (x < rmd ? 1 : 0);
             heap.assign((size_t)std::min<int64_t>(slots, std::max<int64_t>(cnt, 1)), 0);
-            for (int64_t j = 0; j < cnt; ++j) {
+            for (int64_t j = 0; j < cnt && span < best_span; ++j) {
                 // the work group's tasks: the next kStreamWaves of the runs
                 int d = 0;
                 for (int take = kStreamWaves; take > 0 && ri < runs.size();) {
@@ -703,9 +705,9 @@ int stream_batch_cap(const int* shapes, int n, int k, int cus) {
                 }
                 std::pop_heap(heap.begin(), heap.end(), std::greater<int64_t>());
                 heap.back() += d + 4 * k;
+                span = std::max(span, heap.back());
                 std::push_heap(heap.begin(), heap.end(), std::greater<int64_t>());
             }
-            span = std::max(span, *std::max_element(heap.begin(), heap.end()));
         }
         if (span < best_span) {
             best_span = span;

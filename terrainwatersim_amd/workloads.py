@@ -426,11 +426,16 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="Run one workload of this module and print its "
                                              "record as JSON.")
-    ap.add_argument("workload", choices=["small", "batched", "batched_default", "large"])
+    ap.add_argument("workload", choices=["small", "batched", "batched_default", "served", "large"])
     ap.add_argument("--runs", type=int, default=3, help="timed runs (batched_default)")
+    ap.add_argument("--capacity", type=float, default=4.3e11,
+                    help="served: cell-updates/s the arrival rate is a fraction of")
+    ap.add_argument("--requests", type=int, default=200, help="served: requests")
     a = ap.parse_args()
     if a.workload == "batched_default":
         rec = batched_default(runs=a.runs)
+    elif a.workload == "served":
+        rec = served(a.capacity, n_requests=a.requests)
     else:
         rec = globals()[a.workload]()
     print(json.dumps(rec, default=float))

@@ -193,6 +193,10 @@ def batched(n_grids=256, steps=100, seed=3, repeats=3, log=print):
                                       / out["batched_one_launch_cell_updates_per_s"])
     out["batched_over_plain_forced"] = (out["batched_cell_updates_per_s"]
                                         / out["plain_forced_cell_updates_per_s"])
+    out["plain_forced_note"] = ("each grid alone on the tuner's one launch for the whole mix "
+                                "(%d/%d); on the streaming kernel a small grid alone is one "
+                                "wavefront's serial sweep per segment, so this ratio is not "
+                                "the fair one (batched_over_plain is)" % (best["kernel"], best["k"]))
     out["batched_equals_plain"] = bool(out["plain_equals_batched"]
                                        and out["plain_forced_equals_batched"]
                                        and out["layouts_equal"])

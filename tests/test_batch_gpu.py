@@ -420,3 +420,57 @@ def test_auto_class_matches_tiles(gpu, big):
         for j in range(len(shapes)):
             assert np.array_equal(a[j]["depth"], t[j]["depth"]), (r, shapes[j])
             assert np.array_equal(a[j]["flux"], t[j]["flux"]), (r, shapes[j])
+
+
+@pytest.mark.parametrize("launch", [None, (8, 6), (8, 4)])
+def test_changing_grid_sets_match_alone(gpu, launch):
+    """Grids come and go between runs, so the streaming class's segment plan
+    is redone (and the default class switches between the streaming chain and
+    tiles as the streamable cells cross 2^23): after every run each grid
+    equals the same grid stepped alone from its previous state."""
+    def grid(seed, H, W, bd=None, rain=False):
+        b, d, f = state(seed, H, W)
+        p = gpu.params_physical(0.02, 9.81, 1.0, 1.0, 0.995,
+                                gpu.BOUNDARY_CLOSED if bd is None else bd)
+        return [b, d, f, p, (W / 2.0, H / 3.0, 5e-3, 144.0) if rain else None]
+
+    with gpu.Batch(8, 2 * GB) as batch:
+        if launch is not None:
+            batch.set_launch(*launch)
+        live = {}
+
+        def add(g):
+            b, d, f, p, rain = g
+            H, W = b.shape
+            gid = batch.add(W, H, p)
+            batch.upload(gid, b, d, f)
+            if rain:
+                batch.set_rain(gid, *rain)
+            live[gid] = g
+
+        def run(steps):
+            for gid, n in zip(sorted(live), steps):
+                batch.set_steps(gid, n)
+            batch.run()
+            for gid, n in zip(sorted(live), steps):
+                b, d, f, p, rain = live[gid]
+                ref = alone(gpu, b, d, f, p, rain, n)
+                out = batch.download(gid, flux=True)
+                assert np.array_equal(out["depth"], ref["depth"]), (gid, b.shape, n)
+                assert np.array_equal(out["flux"], ref["flux"]), (gid, b.shape, n)
+                live[gid][1], live[gid][2] = out["depth"], out["flux"]
+
+        for g in (grid(1, 2100, 2048, rain=True), grid(2, 2048, 2100), grid(3, 300, 257, rain=True),
+                  grid(4, 129, 700, gpu.BOUNDARY_ZERO), grid(5, 1, 333)):
+            add(g)
+        run([13, 12, 7, 25, 6])
+        # drop both large grids: the default class falls back to tiles
+        big = [gid for gid in sorted(live) if live[gid][0].size > 4_000_000]
+        for gid in big:
+            batch.remove(gid)
+            del live[gid]
+        add(grid(6, 640, 512, rain=True))
+        run([9, 0, 31, 5])
+        # one large grid back (streaming chain again), other steps
+        add(grid(7, 2900, 2900, rain=True))
+        run([100, 3, 1, 17, 23])

@@ -216,6 +216,11 @@ static int class_of(const tws_batch* b, int64_t cells) {
     return n - 1;
 }
 
+// Whether slot s has steps pending in class c of the run.
+static bool pending_in(const tws_batch* b, const Slot& s, int c) {
+    return s.used && s.steps > 0 && class_of(b, (int64_t)s.g.W * s.g.H) == c;
+}
+
 extern "C" {
 
 int64_t tws_grid_footprint(int32_t width, int32_t height) {
@@ -439,7 +444,7 @@ static LaunchClass resolve_class(const tws_batch* b, int c) {
     if (lc.kernel != TWS_KERNEL_AUTO) return lc;
     int64_t cells = 0, all = 0;
     for (const Slot& s : b->slots)
-        if (s.used && s.steps > 0 && class_of(b, (int64_t)s.g.W * s.g.H) == c) {
+        if (pending_in(b, s, c)) {
             all += (int64_t)s.g.W * s.g.H;
             if (stream_usable(s.g)) cells += (int64_t)s.g.W * s.g.H;
         }
@@ -460,9 +465,7 @@ static int chain_cap(tws_batch* b, int c, int k) {
         std::vector<int> shapes;
         for (int i = 0; i < b->max_grids; ++i) {
             const Slot& s = b->slots[i];
-            if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c ||
-                !in_stream_chain(s, k))
-                continue;
+            if (!pending_in(b, s, c) || !in_stream_chain(s, k)) continue;
             shapes.push_back(s.g.W);
             shapes.push_back(s.g.H);
         }
@@ -477,6 +480,65 @@ static int chain_cap(tws_batch* b, int c, int k) {
     } catch (const std::bad_alloc&) {
         return 0;
     }
+}
+
+// Writes header h = {grids, tasks or tiles, table offset, 0}.
+static void set_header(tws_batch* b, int h, int grids, int tasks, int offset) {
+    int* p = b->hdr_host + kHdrInts * h;
+    p[0] = grids;
+    p[1] = tasks;
+    p[2] = offset;
+    p[3] = 0;
+}
+
+// Appends the tail table of streaming class c for remainder r at
+// table_host[*n]: the grids left with r steps after their full launches of
+// lc.k, each entry advanced past those launches (parity) and holding exactly
+// r steps, for one launch of r levels.  *tasks gets the table's task count;
+// *mode is raised to its grids' mode.
+static int append_stream_tail(tws_batch* b, int c, const LaunchClass& lc, int cap, int r, int* n,
+                              int* tasks, int* mode) {
+    const int first = *n;
+    *tasks = 0;
+    for (int i = 0; i < b->max_grids; ++i) {
+        const Slot& s = b->slots[i];
+        if (!pending_in(b, s, c) || s.steps % lc.k != r || !stream_usable(s.g)) continue;
+        GridDesc g = s.g.desc();
+        if (!stream_batch_plan(s.g.W, s.g.H, r, cap, &g.tiles_x, &g.tiles_y, &g.seg_rows))
+            return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
+        g.parity0 ^= (s.steps / lc.k) & 1;
+        g.steps = r;
+        g.tile_begin = *tasks;
+        *tasks += g.tiles_x * g.tiles_y;
+        *mode = std::max(*mode, g.rain_on ? kModeRain : kModePlain);
+        b->table_host[(*n)++] = g;
+    }
+    set_header(b, tail_hdr(c, r), *n - first, *tasks, first);
+    return TWS_OK;
+}
+
+// Appends the fallback tile table of streaming class c at table_host[*n]:
+// its grids the streaming kernel cannot step, as a tile class of kAltKernel
+// at kAltK would step them.  *max_steps, *tiles and *mode describe the table.
+static int append_alt_table(tws_batch* b, int c, int* n, int* max_steps, int* tiles, int* mode) {
+    const int first = *n;
+    *max_steps = *tiles = 0;
+    *mode = kModePlain;
+    for (int i = 0; i < b->max_grids; ++i) {
+        const Slot& s = b->slots[i];
+        if (!pending_in(b, s, c) || stream_usable(s.g)) continue;
+        GridDesc g = s.g.desc();
+        if (!grid_tiles(s.g.W, s.g.H, kAltKernel, kAltK, &g.tiles_x, &g.tiles_y))
+            return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
+        *mode = std::max(*mode, kernel_mode(g));
+        *max_steps = std::max(*max_steps, s.steps);
+        g.tile_begin = *tiles;
+        g.steps = s.steps;
+        *tiles += g.tiles_x * g.tiles_y;
+        b->table_host[(*n)++] = g;
+    }
+    set_header(b, alt_hdr(c), *n - first, *tiles, first);
+    return TWS_OK;
 }
 
 int tws_batch_run(tws_batch* b) {
@@ -497,8 +559,8 @@ int tws_batch_run(tws_batch* b) {
         const int cap = stream ? chain_cap(b, c, lc.k) : 0;
         if (stream && cap <= 0) return fail(TWS_ERR_NOMEM, "tws_batch_run: host allocation failed");
         for (int i = 0; i < b->max_grids; ++i) {
-            Slot& s = b->slots[i];
-            if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c) continue;
+            const Slot& s = b->slots[i];
+            if (!pending_in(b, s, c)) continue;
             GridDesc g = s.g.desc();
             if (stream) {
                 if (stream_usable(s.g) && s.steps % lc.k != 0 && !in_stream_tail(s, lc.k))
@@ -519,69 +581,25 @@ int tws_batch_run(tws_batch* b) {
             tiles += g.tiles_x * g.tiles_y;
             b->table_host[n++] = g;
         }
-        b->hdr_host[kHdrInts * c + 0] = n - first;
-        b->hdr_host[kHdrInts * c + 1] = tiles;
-        b->hdr_host[kHdrInts * c + 2] = first;
-        b->hdr_host[kHdrInts * c + 3] = 0;
+        set_header(b, c, n - first, tiles, first);
         // block bounds rounded up so small changes of the tile (task) count
         // reuse a graph; spare blocks exit
         const int blocks = stream ? ((tiles + 3) / 4 + 63) / 64 * 64 : (tiles + 511) / 512 * 512;
-        // a streaming class's tails: per remainder r, a table of the grids
-        // left with r steps, each entry advanced past the full launches
-        // (parity) and holding exactly r steps, for one launch of r levels
+        // a streaming class's tails (one launch per remainder r) and its
+        // fallback tile table
         int tails = 0, tail_blocks = 0;
-        for (int r = 1; stream && r < lc.k && r <= kMaxTail; ++r) {
-            const int tfirst = n;
-            int tasks = 0;
-            for (int i = 0; i < b->max_grids; ++i) {
-                Slot& s = b->slots[i];
-                if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c ||
-                    s.steps % lc.k != r || !stream_usable(s.g))
-                    continue;
-                GridDesc g = s.g.desc();
-                if (!stream_batch_plan(s.g.W, s.g.H, r, cap, &g.tiles_x, &g.tiles_y, &g.seg_rows))
-                    return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
-                g.parity0 ^= (s.steps / lc.k) & 1;
-                g.steps = r;
-                g.tile_begin = tasks;
-                tasks += g.tiles_x * g.tiles_y;
-                mode = std::max(mode, g.rain_on ? kModeRain : kModePlain);
-                b->table_host[n++] = g;
-            }
-            int* h = b->hdr_host + kHdrInts * tail_hdr(c, r);
-            h[0] = n - tfirst;
-            h[1] = tasks;
-            h[2] = tfirst;
-            h[3] = 0;
-            if (n == tfirst) continue;
-            tails |= 1 << r;
-            tail_blocks = std::max(tail_blocks, ((tasks + 3) / 4 + 63) / 64 * 64);
-        }
-        // a streaming class's grids the streaming kernel cannot step: the
-        // fallback tile launch, as a tile class would step them
         int alt_steps = 0, alt_tiles = 0, alt_mode = kModePlain;
         if (stream) {
-            const int afirst = n;
-            for (int i = 0; i < b->max_grids; ++i) {
-                Slot& s = b->slots[i];
-                if (!s.used || s.steps <= 0 || class_of(b, (int64_t)s.g.W * s.g.H) != c ||
-                    stream_usable(s.g))
-                    continue;
-                GridDesc g = s.g.desc();
-                if (!grid_tiles(s.g.W, s.g.H, kAltKernel, kAltK, &g.tiles_x, &g.tiles_y))
-                    return fail(TWS_ERR_INVALID, "tws_batch_run: bad launch config");
-                alt_mode = std::max(alt_mode, kernel_mode(g));
-                alt_steps = std::max(alt_steps, s.steps);
-                g.tile_begin = alt_tiles;
-                g.steps = s.steps;
-                alt_tiles += g.tiles_x * g.tiles_y;
-                b->table_host[n++] = g;
+            for (int r = 1; r < lc.k && r <= kMaxTail; ++r) {
+                int tasks = 0;
+                const int rc = append_stream_tail(b, c, lc, cap, r, &n, &tasks, &mode);
+                if (rc != TWS_OK) return rc;
+                if (tasks == 0) continue;
+                tails |= 1 << r;
+                tail_blocks = std::max(tail_blocks, ((tasks + 3) / 4 + 63) / 64 * 64);
             }
-            int* h = b->hdr_host + kHdrInts * alt_hdr(c);
-            h[0] = n - afirst;
-            h[1] = alt_tiles;
-            h[2] = afirst;
-            h[3] = 0;
+            const int rc = append_alt_table(b, c, &n, &alt_steps, &alt_tiles, &alt_mode);
+            if (rc != TWS_OK) return rc;
         }
         if (max_steps == 0 && tails == 0 && alt_steps == 0) continue;
         chains.push_back(Chain{c, lc.kernel, lc.k, (max_steps + lc.k - 1) / lc.k, blocks, mode,

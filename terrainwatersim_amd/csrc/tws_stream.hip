@@ -20,7 +20,9 @@
 //     a level's rows are read while the level before it computes;
 //   - each head difference serves the two cells it separates (negated for
 //     the second: fl(a - b) = -fl(b - a));
-//   - the input rows are loaded two row-steps ahead;
+//   - the input rows are loaded two row-steps ahead, and the new state is
+//     stored with the default cache policy (stores share the memory counter
+//     that the row-steps' waits on those loads read);
 //   - the two wavefronts sharing a SIMD take turns at the higher issue
 //     priority, so neither runs alone at the end of the launch.
 // The x-halo is the tile kernels': HALOX = 2 KH columns on each side go stale

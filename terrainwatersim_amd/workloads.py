@@ -276,7 +276,7 @@ def _pcts(lat):
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
 
 
-def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=100, log=print,
+def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=96, log=print,
            device_cap=None):
     p = tws.params_physical(**PARAMS)
     probe = Requests(n_requests, seed, 1.0)

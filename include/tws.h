@@ -441,7 +441,7 @@ typedef struct tws_server_config {
     int64_t pool_bytes;        /* device bytes for resident request state     */
     int32_t max_resident;      /* requests on the device at once              */
     int32_t chunk_steps;       /* steps per batched run (admission latency;
-                                  0 = 96, whole launches of the batch default) */
+                                  0 = 48, whole launches of the batch default) */
     int32_t kernel;            /* TWS_KERNEL_* for the batch (0 = auto)       */
     int32_t steps_per_launch;  /* 0 = auto                                    */
     int32_t max_queue;         /* queued requests before submit refuses       */

@@ -459,7 +459,7 @@ class Server:
     """Continuous-batching request loop (tws_server_*).  Inputs are copied at
     submit; output arrays are kept alive here until the request finishes."""
 
-    def __init__(self, pool_bytes: int, max_resident: int = 64, chunk_steps: int = 96,
+    def __init__(self, pool_bytes: int, max_resident: int = 64, chunk_steps: int = 48,
                  kernel: int = 0, steps_per_launch: int = 0, max_queue: int = 0):
         cfg = ServerConfig(int(pool_bytes), int(max_resident), int(chunk_steps), int(kernel),
                            int(steps_per_launch), int(max_queue))

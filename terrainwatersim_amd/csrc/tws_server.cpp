@@ -354,12 +354,13 @@ int tws_server_create(const tws_server_config* config, tws_server** out) {
     *out = nullptr;
     tws_server_config c = *config;
     if (c.max_resident <= 0) c.max_resident = 64;
-    // 96: a whole number of launches of the batch default (6 steps per
+    // 48: a whole number of launches of the batch default (6 steps per
     // streaming launch, 4 or 8 per tile launch), so a chunk ends in no tail
-    // launch.  MI355X, the served mix saturated: 4.31-4.33e11 cell-updates/s
-    // against 4.28-4.29e11 at 100 steps; at 70% load p50 0.141-0.143 s
-    // against 0.145 s.
-    if (c.chunk_steps <= 0) c.chunk_steps = 96;
+    // launch, and short enough that arrivals join soon.  MI355X, the served
+    // mix at 70% load, p50: 0.136-0.140 s at 48 steps, 0.142 s at 96, 0.151 s
+    // at 192; saturated 4.40e11 cell-updates/s at 48 against 4.36-4.38e11 at
+    // 96 and 4.28-4.29e11 at 100 (which ends in a tail launch).
+    if (c.chunk_steps <= 0) c.chunk_steps = 48;
     if (c.max_queue <= 0) c.max_queue = 1 << 16;
     if (c.pool_bytes <= 0) return fail(TWS_ERR_INVALID, "tws_server_create: pool_bytes must be > 0");
     tws_server* s = new (std::nothrow) tws_server();

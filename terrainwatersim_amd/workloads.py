@@ -283,7 +283,7 @@ def _pcts(lat):
 MAX_RESIDENT = 128
 
 
-def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=96, log=print,
+def served(capacity_cups, n_requests=200, seed=4, load=0.7, chunk_steps=48, log=print,
            device_cap=None):
     p = tws.params_physical(**PARAMS)
     probe = Requests(n_requests, seed, 1.0)

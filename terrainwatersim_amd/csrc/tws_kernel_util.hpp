@@ -142,10 +142,8 @@ __device__ __forceinline__ void flux_update_p(float F, float C, float d, float p
     // flowUpdate.comp:57 summed pairwise, the order Mesa executes the reference in
     const float total = ((o0 + o1) + (o2 + o3)) * C;
     if (total > d) {
-        // d == 0 (a dry cell on a slope) gives s = 0/total = +0 exactly, so the
-        // division is only executed where some lane holds water.
-        float s = 0.0f;
-        if (d != 0.0f) s = d / total;
+        // flowUpdate.comp:58-59 (d == 0, a dry cell on a slope, gives +0)
+        const float s = d / total;
         o0 = o0 * s; o1 = o1 * s; o2 = o2 * s; o3 = o3 * s;
     }
     f0 = o0; f1 = o1; f2 = o2; f3 = o3;

@@ -431,10 +431,13 @@ k_stream(GridDesc g1, const GridDesc* __restrict__ grids, const int* __restrict_
             // flux of row a: flowUpdate.comp:34-62.  x products: +x of column
             // c, whose negation is -x of column c+1 (the last column's from
             // the right lane's first head; the first column's -x from the
-            // left lane's last +x product).  Lanes at the strip's edges take
-            // their own head as the missing neighbour (zero difference).
+            // left lane's last +x product).  Lanes at the strip's edges read
+            // 0 for the missing neighbour: their values are wrong whatever
+            // the fill, and reach 2 columns further per level, i.e. the
+            // 2 KH halo columns and no output column.  (A zero fill lets the
+            // lane shift fold into the subtraction.)
             {
-                const float h_right = from_right_lane_or(h0[0], h0[kSCPL - 1]);
+                const float h_right = from_right_lane(h0[0]);
                 float px[kSCPL];
 #pragma unroll
                 for (int c = 0; c < kSCPL; ++c)

@@ -1,6 +1,7 @@
 """Batched form: many grids in one launch sequence, bit-identical to each grid
 stepped alone (and so to the oracle and the reference shaders)."""
 import numpy as np
+from bitwise import same_bits
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -63,8 +64,8 @@ def test_batch_equals_alone(gpu, graphs):
         for gid, (b, d, f, p, rain) in zip(ids, grids):
             out = batch.download(gid)
             ref = alone(gpu, b, d, f, p, rain, 13)
-            assert np.array_equal(out["depth"], ref["depth"]), b.shape
-            assert np.array_equal(out["flux"], ref["flux"]), b.shape
+            assert same_bits(out["depth"], ref["depth"]), b.shape
+            assert same_bits(out["flux"], ref["flux"]), b.shape
             assert batch.check_finite(gid)
 
 
@@ -79,7 +80,7 @@ def test_per_grid_step_counts(gpu):
         for gid, n, (b, d, f, p, rain) in zip(ids, counts, grids):
             ref = alone(gpu, b, d, f, p, rain, n)
             out = batch.download(gid)
-            assert np.array_equal(out["depth"], ref["depth"]) and np.array_equal(out["flux"], ref["flux"])
+            assert same_bits(out["depth"], ref["depth"]) and same_bits(out["flux"], ref["flux"])
 
 
 def test_graph_replays_match_eager(gpu):
@@ -95,7 +96,7 @@ def test_graph_replays_match_eager(gpu):
             e_batch.step(8)
             g_batch.sync()
             for a, c in zip(gi, ei):
-                assert np.array_equal(g_batch.download(a)["depth"], e_batch.download(c)["depth"]), rep
+                assert same_bits(g_batch.download(a)["depth"], e_batch.download(c)["depth"]), rep
         st = g_batch.stats()
         assert st["graph_captures"] == 1 and st["graph_replays"] == 4
 
@@ -107,7 +108,7 @@ def test_chunks_equal_one_run(gpu):
         for _ in range(5):
             batch.step(7)
         for gid, (b, d, f, p, rain) in zip(ids, grids):
-            assert np.array_equal(batch.download(gid)["depth"], alone(gpu, b, d, f, p, rain, 35)["depth"])
+            assert same_bits(batch.download(gid)["depth"], alone(gpu, b, d, f, p, rain, 35)["depth"])
 
 
 def test_capacity_and_reuse(gpu):
@@ -142,7 +143,7 @@ def test_nan_in_one_grid_is_isolated(gpu):
         assert batch.check_finite(ids[0]) and batch.check_finite(ids[2])
         assert not batch.check_finite(ids[1])
         b, d, f, p, rain = grids[0]
-        assert np.array_equal(batch.download(ids[0])["depth"], alone(gpu, b, d, f, p, rain, 5)["depth"])
+        assert same_bits(batch.download(ids[0])["depth"], alone(gpu, b, d, f, p, rain, 5)["depth"])
 
 
 def test_workload_grids_vs_restatements(gpu, oracle):
@@ -169,7 +170,7 @@ def test_workload_grids_vs_restatements(gpu, oracle):
     for (s, b, rain), out in zip(grids, outs):
         d0 = np.zeros((s, s), F32)
         d32, f32, _ = oracle.step(b, d0, None, F, K, Cc, n=100, rain=rain)
-        assert np.array_equal(out["depth"], d32) and np.array_equal(out["flux"], f32), s
+        assert same_bits(out["depth"], d32) and same_bits(out["flux"], f32), s
         d64, _, _ = oracle.step(b, d0, None, F, K, Cc, n=100, rain=rain, dtype=np.float64)
         m = float(d64.max())
         err = np.abs(out["depth"].astype(np.float64) - d64)
@@ -271,8 +272,8 @@ def test_launch_classes_bitwise(gpu, oracle, graphs):
                                           rain=rain)
                 want[j] = (d32, f32)
                 out = batch.download(gid, flux=True)
-                assert np.array_equal(out["depth"], d32), (run, CLASS_SHAPES[j])
-                assert np.array_equal(out["flux"], f32), (run, CLASS_SHAPES[j])
+                assert same_bits(out["depth"], d32), (run, CLASS_SHAPES[j])
+                assert same_bits(out["flux"], f32), (run, CLASS_SHAPES[j])
         st = batch.stats()
         if graphs:
             assert st["graph_captures"] == 1 and st["graph_replays"] == 3
@@ -335,8 +336,8 @@ def test_launch_classes_with_stream_bitwise(gpu, oracle, graphs):
                                           rain=rain)
                 want[j] = (d32, f32)
                 out = batch.download(gid, flux=True)
-                assert np.array_equal(out["depth"], d32), (run, STREAM_CLASS_SHAPES[j])
-                assert np.array_equal(out["flux"], f32), (run, STREAM_CLASS_SHAPES[j])
+                assert same_bits(out["depth"], d32), (run, STREAM_CLASS_SHAPES[j])
+                assert same_bits(out["flux"], f32), (run, STREAM_CLASS_SHAPES[j])
                 assert batch.check_finite(gid)
 
 
@@ -383,8 +384,8 @@ def test_stream_chain_bitwise(gpu, oracle, k, graphs):
                                           rain=rain)
                 want[j] = (d32, f32)
                 out = batch.download(gid, flux=True)
-                assert np.array_equal(out["depth"], d32), (run, CHAIN_SHAPES[j])
-                assert np.array_equal(out["flux"], f32), (run, CHAIN_SHAPES[j])
+                assert same_bits(out["depth"], d32), (run, CHAIN_SHAPES[j])
+                assert same_bits(out["flux"], f32), (run, CHAIN_SHAPES[j])
 
 
 @pytest.mark.parametrize("big", [True, False])
@@ -418,8 +419,8 @@ def test_auto_class_matches_tiles(gpu, big):
             out[name] = res
     for r, (a, t) in enumerate(zip(out["auto"], out["tiles"])):
         for j in range(len(shapes)):
-            assert np.array_equal(a[j]["depth"], t[j]["depth"]), (r, shapes[j])
-            assert np.array_equal(a[j]["flux"], t[j]["flux"]), (r, shapes[j])
+            assert same_bits(a[j]["depth"], t[j]["depth"]), (r, shapes[j])
+            assert same_bits(a[j]["flux"], t[j]["flux"]), (r, shapes[j])
 
 
 @pytest.mark.parametrize("launch", [None, (8, 6), (8, 4)])
@@ -456,8 +457,8 @@ def test_changing_grid_sets_match_alone(gpu, launch):
                 b, d, f, p, rain = live[gid]
                 ref = alone(gpu, b, d, f, p, rain, n)
                 out = batch.download(gid, flux=True)
-                assert np.array_equal(out["depth"], ref["depth"]), (gid, b.shape, n)
-                assert np.array_equal(out["flux"], ref["flux"]), (gid, b.shape, n)
+                assert same_bits(out["depth"], ref["depth"]), (gid, b.shape, n)
+                assert same_bits(out["flux"], ref["flux"]), (gid, b.shape, n)
                 live[gid][1], live[gid][2] = out["depth"], out["flux"]
 
         for g in (grid(1, 2100, 2048, rain=True), grid(2, 2048, 2100), grid(3, 300, 257, rain=True),

@@ -10,6 +10,7 @@ import json
 from pathlib import Path
 
 import numpy as np
+from bitwise import same_bits
 import pytest
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -51,9 +52,9 @@ def test_oracle_matches_reference_shaders(oracle, name):
     for n in case["steps"]:
         d, f, v = oracle.step(z["b"], z["d0"], z.get("f0"), F, K, Cc, n=n, boundary=boundary,
                               rain=case["rain"], velocity=True)
-        assert np.array_equal(d, z[f"d_{n}"]), f"{name}: depth differs after {n} steps"
-        assert np.array_equal(f, z[f"f_{n}"]), f"{name}: flux differs after {n} steps"
-        assert np.array_equal(half_rtz_numpy(v), z[f"v16_{n}"]), f"{name}: FlowMap differs"
+        assert same_bits(d, z[f"d_{n}"]), f"{name}: depth differs after {n} steps"
+        assert same_bits(f, z[f"f_{n}"]), f"{name}: flux differs after {n} steps"
+        assert same_bits(half_rtz_numpy(v), z[f"v16_{n}"]), f"{name}: FlowMap differs"
 
 
 def test_half_rtz_library_matches_numpy(tws):
@@ -63,7 +64,7 @@ def test_half_rtz_library_matches_numpy(tws):
         np.array([0.0, -0.0, 1.0, -1.0, 65504.0, 65519.0, 65520.0, 70000.0, -70000.0,
                   6.1e-5, 5.96e-8, 2.9e-8, 1e-40, np.inf, -np.inf], F32),
     ])
-    assert np.array_equal(tws.float_to_half_rtz(x), half_rtz_numpy(x))
+    assert same_bits(tws.float_to_half_rtz(x), half_rtz_numpy(x))
     nan = tws.float_to_half_rtz(np.array([np.nan], F32))
     assert np.isnan(nan.view(np.float16)[0])
 
@@ -76,11 +77,11 @@ def test_fixtures_are_not_trivial():
     assert z["d_20"].sum(dtype=np.float64) < z["d0"].sum(dtype=np.float64)  # drained
     # the reference's dispatch shapes leave the tail of a 100 x 88 grid as it was
     z = load("random100x88_dispatch")
-    assert np.array_equal(z["d_25"][80:], z["d0"][80:])
-    assert np.array_equal(z["d_25"][:, 96:], z["d0"][:, 96:])
-    assert np.array_equal(z["f_25"][:, 96:], z["f0"][:, 96:])
+    assert same_bits(z["d_25"][80:], z["d0"][80:])
+    assert same_bits(z["d_25"][:, 96:], z["d0"][:, 96:])
+    assert same_bits(z["f_25"][:, 96:], z["f0"][:, 96:])
     assert not z["v16_25"][:, 96:].any()
-    assert not np.array_equal(z["d_25"][64:80, :96], z["d0"][64:80, :96])
+    assert not same_bits(z["d_25"][64:80, :96], z["d0"][64:80, :96])
 
 
 WINDOWS = sorted(MANIFEST.get("window_cases", {}))
@@ -111,8 +112,8 @@ def test_oracle_matches_reference_window(tws, oracle, name):
     d, f, v = oracle.step(bs, np.full(bs.shape, 0.5, F32), None, case["F_value"],
                           case["K_value"], case["C_value"], n=n, velocity=True)
     w = (slice(m, m + wh), slice(m, m + ww))
-    assert np.array_equal(bs[w], z["b_window"])
-    assert np.array_equal(d[w], z[f"d_{n}"])
-    assert np.array_equal(f[w], z[f"f_{n}"])
-    assert np.array_equal(half_rtz_numpy(v[w]), z[f"v16_{n}"])
-    assert z[f"f_{n}"].any() and not np.array_equal(z[f"d_{n}"], np.full((wh, ww), 0.5, F32))
+    assert same_bits(bs[w], z["b_window"])
+    assert same_bits(d[w], z[f"d_{n}"])
+    assert same_bits(f[w], z[f"f_{n}"])
+    assert same_bits(half_rtz_numpy(v[w]), z[f"v16_{n}"])
+    assert z[f"f_{n}"].any() and not same_bits(z[f"d_{n}"], np.full((wh, ww), 0.5, F32))

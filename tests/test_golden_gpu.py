@@ -8,6 +8,7 @@ import json
 from pathlib import Path
 
 import numpy as np
+from bitwise import same_bits
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -38,9 +39,9 @@ def test_hip_matches_reference_shaders(gpu, name, kernel, k):
             s.step(n)
             out = s.download(flux=True)
             ref = s.download_reference_layout()
-        assert np.array_equal(out["depth"], z[f"d_{n}"]), f"{name}: depth after {n} steps"
-        assert np.array_equal(out["flux"], z[f"f_{n}"]), f"{name}: flux after {n} steps"
-        assert np.array_equal(ref["flowmap_rg16f"], z[f"v16_{n}"]), f"{name}: FlowMap"
+        assert same_bits(out["depth"], z[f"d_{n}"]), f"{name}: depth after {n} steps"
+        assert same_bits(out["flux"], z[f"f_{n}"]), f"{name}: flux after {n} steps"
+        assert same_bits(ref["flowmap_rg16f"], z[f"v16_{n}"]), f"{name}: FlowMap"
 
 
 WINDOWS = sorted(MANIFEST.get("window_cases", {}))
@@ -69,7 +70,7 @@ def test_hip_matches_reference_window(gpu, name, kernel, k):
         out = s.download(flux=True)
         ref = s.download_reference_layout()
     w = (slice(y0, y0 + wh), slice(x0, x0 + ww))
-    assert np.array_equal(b[w], z["b_window"])
-    assert np.array_equal(out["depth"][w], z[f"d_{n}"]), "depth window differs"
-    assert np.array_equal(out["flux"][w], z[f"f_{n}"]), "flux window differs"
-    assert np.array_equal(ref["flowmap_rg16f"][w], z[f"v16_{n}"]), "FlowMap window differs"
+    assert same_bits(b[w], z["b_window"])
+    assert same_bits(out["depth"][w], z[f"d_{n}"]), "depth window differs"
+    assert same_bits(out["flux"][w], z[f"f_{n}"]), "flux window differs"
+    assert same_bits(ref["flowmap_rg16f"][w], z[f"v16_{n}"]), "FlowMap window differs"

@@ -1,6 +1,7 @@
 """Device-side initial state: the fBm terrain computed on the GPU equals the
 host generator bit for bit; the water initialiser writes the sheet/column."""
 import numpy as np
+from bitwise import same_bits
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -15,7 +16,7 @@ def test_device_fbm_equals_host(gpu, shape, seed, octaves, base):
     with gpu.Simulation(W, H) as s:
         s.init_terrain_fbm(seed, octaves, base, 0.43, 100.0)
         ref = s.download_reference_layout()
-    assert np.array_equal(ref["terrain_rgba"][..., 0], host)
+    assert same_bits(ref["terrain_rgba"][..., 0], host)
 
 
 def test_init_water_and_step(gpu, oracle):
@@ -30,10 +31,10 @@ def test_init_water_and_step(gpu, oracle):
         out = s.download(flux=True)
     d0 = np.full((H, W), 0.25, np.float32)
     d0[50:90, 100:140] = 20.0
-    assert np.array_equal(out0["depth"], d0) and not out0["flux"].any()
+    assert same_bits(out0["depth"], d0) and not out0["flux"].any()
     de, fe, _ = oracle.step(b, d0, None, p.flow_friction, p.water_acceleration,
                             p.cell_area_inv_timescaled, n=17)
-    assert np.array_equal(out["depth"], de) and np.array_equal(out["flux"], fe)
+    assert same_bits(out["depth"], de) and same_bits(out["flux"], fe)
 
 
 def test_batch_device_init(gpu):
@@ -60,8 +61,8 @@ def test_device_reference_scene_equals_restatement(gpu, N, scene_index, height_s
         s.init_reference_scene(height_scale, R_SEED, scene_index)
         t = s.download_reference_layout()["terrain_rgba"]
         f = s.download(flux=True)["flux"]
-    assert np.array_equal(t[..., 0], b0)
-    assert np.array_equal(t[..., 3], d0)
+    assert same_bits(t[..., 0], b0)
+    assert same_bits(t[..., 3], d0)
     assert not f.any()
 
 
@@ -71,8 +72,8 @@ def test_device_reference_scene_equals_host_at_4096(gpu):
     with gpu.Simulation(N, N) as s:
         s.init_reference_scene()
         t = s.download_reference_layout()["terrain_rgba"]
-    assert np.array_equal(t[..., 0], b0)
-    assert np.array_equal(t[..., 3], d0)
+    assert same_bits(t[..., 0], b0)
+    assert same_bits(t[..., 3], d0)
 
 
 def test_device_reference_scene_resets_a_stepped_grid(gpu, oracle):
@@ -92,12 +93,12 @@ def test_device_reference_scene_resets_a_stepped_grid(gpu, oracle):
             out0 = s.download(flux=True)
             s.step(12)
             out = s.download(flux=True, velocity=True)
-        assert np.array_equal(out0["depth"], d) and not out0["flux"].any()
+        assert same_bits(out0["depth"], d) and not out0["flux"].any()
         de, fe, ve = oracle.step(b, d, None, p.flow_friction, p.water_acceleration,
                                  p.cell_area_inv_timescaled, n=12,
                                  boundary=oracle.BOUNDARY_ZERO, velocity=True)
-        assert np.array_equal(out["depth"], de) and np.array_equal(out["flux"], fe)
-        assert np.array_equal(out["velocity"], ve)
+        assert same_bits(out["depth"], de) and same_bits(out["flux"], fe)
+        assert same_bits(out["velocity"], ve)
 
 
 def test_device_reference_scene_rejects_bad_grids(gpu):
@@ -123,4 +124,4 @@ def test_batch_device_reference_scene(gpu):
         with pytest.raises(gpu.TwsError):
             batch.init_reference_scene(other)
         d = batch.download(g)
-    assert np.array_equal(d["depth"], d0) and not d["flux"].any()
+    assert same_bits(d["depth"], d0) and not d["flux"].any()

@@ -7,6 +7,7 @@ properties: volume conservation, non-negativity, blocked == unblocked, and a
 bitwise comparison of the first steps against the oracle.
 """
 import numpy as np
+from bitwise import same_bits
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -42,10 +43,10 @@ def p3(p):
 
 
 def assert_same(out, d, f, v=None):
-    assert np.array_equal(out["depth"], d), f"depth differs at {np.argwhere(out['depth'] != d)[:5]}"
-    assert np.array_equal(out["flux"], f), "flux differs"
+    assert same_bits(out["depth"], d), f"depth differs at {np.argwhere(out['depth'] != d)[:5]}"
+    assert same_bits(out["flux"], f), "flux differs"
     if v is not None:
-        assert np.array_equal(out["velocity"], v), "velocity differs"
+        assert same_bits(out["velocity"], v), "velocity differs"
 
 
 def test_lane_shift_layout(gpu):
@@ -92,8 +93,27 @@ def test_rain_disc_positions(gpu, oracle, kernel, k):
     for rain in RAIN_DISCS:
         out = run_gpu(gpu, b, d, f, 9, kernel, k, p, rain=rain, velocity=False)
         de, fe, _ = oracle.step(b, d, f, *p3(p), n=9, rain=rain)
-        assert np.array_equal(out["depth"], de), rain
-        assert np.array_equal(out["flux"], fe), rain
+        assert same_bits(out["depth"], de), rain
+        assert same_bits(out["flux"], fe), rain
+
+
+@pytest.mark.parametrize("kernel,k", [(1, 1), (3, 5), (5, 8), (7, 8), (8, 6), (8, 4)])
+def test_negative_zero_depth_bitwise(gpu, oracle, kernel, k):
+    """Cells uploaded with a depth of -0 on a slope: their outflow exceeds
+    their depth, so the flux is scaled by -0 / total = -0 (flowUpdate.comp:
+    58-59), which signs the scaled fluxes.  One step, i.e. each kernel
+    family's one-step form, compared by bit pattern.  The -0 cells have no
+    -0 neighbour, so no depth update meets max(0, -0) (flowApply.comp:41),
+    whose sign of zero differs between implementations of max."""
+    b, d, f = random_state(90 + kernel * 10 + k, 130, 250)
+    rng = np.random.default_rng(kernel * 100 + k)
+    y, x = np.indices(d.shape)
+    d[((x + 2 * y) % 5 == 0) & (rng.random(d.shape) < 0.7)] = F32(-0.0)
+    p = gpu.params_physical(**CFG_PARAMS)
+    out = run_gpu(gpu, b, d, f, 1, kernel, k, p)
+    de, fe, ve = oracle.step(b, d, f, *p3(p), n=1, velocity=True)
+    assert np.signbit(fe).any() and not np.signbit(de).any()  # the case the test is for
+    assert_same(out, de, fe, ve)
 
 
 def test_twopass_bitwise(gpu, oracle):
@@ -227,7 +247,7 @@ def test_full_size_4096_window_vs_restatements(gpu, oracle):
     ds = np.ascontiguousarray(d[s0:s1, s0:s1])
     oracle.set_threads(16)
     d32, _, _ = oracle.step(bs, ds, None, *p3(p), n=t)
-    assert np.array_equal(win, d32[pad:pad + window, pad:pad + window])
+    assert same_bits(win, d32[pad:pad + window, pad:pad + window])
     d64, _, _ = oracle.step(bs, ds, None, *p3(p), n=t, dtype=np.float64)
     err, _, m = fp64_report(f"4096^2 sheet, centre {window}^2 after {t} steps", win,
                             d64[pad:pad + window, pad:pad + window])
@@ -271,8 +291,8 @@ def test_config2_full_domain_vs_restatements(gpu, oracle, capsys):
         return dd, ff
 
     d32, f32 = restate(np.float32)
-    assert np.array_equal(out["depth"], d32)
-    assert np.array_equal(out["flux"], f32)
+    assert same_bits(out["depth"], d32)
+    assert same_bits(out["flux"], f32)
     del d32, f32
     d64, _ = restate(np.float64)
     err = np.abs(out["depth"].astype(np.float64) - d64)
@@ -311,7 +331,7 @@ def test_brush_once(gpu, oracle):
         s.upload(b, d, f)
         s.apply_brush(33.5, 47.25, 0.02, 32.0)
         out = s.download(flux=False)
-    assert np.array_equal(out["depth"], oracle.brush(d, 33.5, 47.25, 0.02, 32.0))
+    assert same_bits(out["depth"], oracle.brush(d, 33.5, 47.25, 0.02, 32.0))
 
 
 def test_reference_layout_roundtrip(gpu, oracle):
@@ -326,12 +346,12 @@ def test_reference_layout_roundtrip(gpu, oracle):
         out = s.download(flux=True, velocity=True)
     de, fe, ve = oracle.step(b, d, f, *p3(p), n=5, velocity=True)
     assert_same(out, de, fe, ve)
-    assert np.array_equal(ref["terrain_rgba"][..., 0], b)
-    assert np.array_equal(ref["terrain_rgba"][..., 3], de)
+    assert same_bits(ref["terrain_rgba"][..., 0], b)
+    assert same_bits(ref["terrain_rgba"][..., 3], de)
     assert (ref["terrain_rgba"][..., 1:3] == F32(0.3)).all()
-    assert np.array_equal(ref["flow_rgba"], fe)
+    assert same_bits(ref["flow_rgba"], fe)
     # rg16f store rounds toward zero (the Mesa behaviour the fixtures pin)
-    assert np.array_equal(ref["flowmap_rg16f"], gpu.float_to_half_rtz(ve))
+    assert same_bits(ref["flowmap_rg16f"], gpu.float_to_half_rtz(ve))
 
 
 def test_nonfinite_input_is_flagged(gpu):
@@ -402,7 +422,7 @@ def test_reference_dispatch_bitwise(gpu, oracle, shape, rain):
     if bw and bh:
         want[:bh, :bw] = oracle.brush(np.ascontiguousarray(d[:bh, :bw]), W - 3.5, H - 2.25, 0.5,
                                       64.0)
-    assert np.array_equal(got, want)
+    assert same_bits(got, want)
 
 
 @pytest.mark.parametrize("kernel,k", [(1, 1), (2, 4), (3, 5), (4, 8), (5, 6), (6, 3), (7, 8),

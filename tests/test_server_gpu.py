@@ -2,6 +2,7 @@
 the same solve run alone (bitwise); arrivals join a running batch; failure
 handling is per request."""
 import numpy as np
+from bitwise import same_bits
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -43,7 +44,7 @@ def test_results_equal_alone(gpu):
             assert st.state == gpu.REQ_DONE and st.steps_done == n
             assert st.t_submit <= st.t_admit <= st.t_done
             ref = alone(gpu, b, d, n, p, rain)
-            assert np.array_equal(out["depth"], ref["depth"]) and np.array_equal(out["flux"], ref["flux"])
+            assert same_bits(out["depth"], ref["depth"]) and same_bits(out["flux"], ref["flux"])
         stats = srv.stats()
         assert stats["completed"] == len(jobs) and stats["failed"] == 0
         assert stats["cell_updates"] == sum(H * W * n for _, H, W, n, _ in jobs)
@@ -65,12 +66,12 @@ def test_arrivals_join_running_batch(gpu):
         for t, out, bs, ds in shorts:
             st = srv.wait(t)
             assert st.state == gpu.REQ_DONE
-            assert np.array_equal(out["depth"], alone(gpu, bs, ds, 40, p)["depth"])
+            assert same_bits(out["depth"], alone(gpu, bs, ds, 40, p)["depth"])
         st_long = srv.wait(t_long, timeout=300)
         assert st_long.state == gpu.REQ_DONE
         assert all(srv.poll(t).t_done < st_long.t_done for t, *_ in shorts)
         assert all(srv.poll(t).t_admit > srv.poll(t_long).t_admit for t, *_ in shorts)
-        assert np.array_equal(out_long["depth"], alone(gpu, b, d, 3000, p)["depth"])
+        assert same_bits(out_long["depth"], alone(gpu, b, d, 3000, p)["depth"])
 
 
 def test_admission_by_footprint(gpu):
@@ -109,7 +110,7 @@ def test_cancel_and_nonfinite_are_isolated(gpu):
         st_bad = srv.poll(t_bad)
         assert st_bad.state == gpu.REQ_FAILED and st_bad.error == -4
         assert srv.poll(t_cancel).state == gpu.REQ_CANCELLED
-        assert np.array_equal(out_ok["depth"], alone(gpu, b, d, 60, p)["depth"])
+        assert same_bits(out_ok["depth"], alone(gpu, b, d, 60, p)["depth"])
 
 
 def test_release_forgets_finished_requests_only(gpu):
@@ -156,7 +157,7 @@ def test_workload_requests_vs_restatements(gpu, oracle):
             n = int(req.steps[i])
             assert srv.poll(t).state == gpu.REQ_DONE
             d32, f32, _ = oracle.step(b, d, None, F, K, Cc, n=n, rain=rain)
-            assert np.array_equal(out["depth"], d32) and np.array_equal(out["flux"], f32), i
+            assert same_bits(out["depth"], d32) and same_bits(out["flux"], f32), i
             d64, _, _ = oracle.step(b, d, None, F, K, Cc, n=n, rain=rain, dtype=np.float64)
             m = float(d64.max())
             err = np.abs(out["depth"].astype(np.float64) - d64)

@@ -142,7 +142,9 @@ __device__ __forceinline__ void flux_update_p(float F, float C, float d, float p
     // flowUpdate.comp:57 summed pairwise, the order Mesa executes the reference in
     const float total = ((o0 + o1) + (o2 + o3)) * C;
     if (total > d) {
-        // flowUpdate.comp:58-59 (d == 0, a dry cell on a slope, gives +0)
+        // flowUpdate.comp:58-59.  A dry cell on a slope (d = +0) gets s = +0;
+        // an uploaded d = -0 gets s = -0 and signs the scaled fluxes, as the
+        // division in the shader does.
         const float s = d / total;
         o0 = o0 * s; o1 = o1 * s; o2 = o2 * s; o3 = o3 * s;
     }

@@ -70,10 +70,16 @@ void oracle_params_physical(float dt, float gravity, float cell_size, float pipe
     out[2] = (float)((double)dt / ((double)cell_size * (double)cell_size));
 }
 
+/* GLSL's max (GLSL 4.60 §8.3): "y if x < y, otherwise x".  The shaders call
+ * it as max(0, t) (flowUpdate.comp:54, flowApply.comp:41), which gives +0 for
+ * t = -0; glibc's fmaxf returns -0 there.  For t = NaN both give 0. */
+static inline float glsl_maxf(float x, float y) { return x < y ? y : x; }
+static inline double glsl_max(double x, double y) { return x < y ? y : x; }
+
 /* ------------------------------------------------------------------------- */
 #define ORACLE_REAL float
 #define ORACLE_SFX f32
-#define ORACLE_MAX fmaxf
+#define ORACLE_MAX glsl_maxf
 #include "tws_oracle_body.h"
 #undef ORACLE_REAL
 #undef ORACLE_SFX
@@ -81,7 +87,7 @@ void oracle_params_physical(float dt, float gravity, float cell_size, float pipe
 
 #define ORACLE_REAL double
 #define ORACLE_SFX f64
-#define ORACLE_MAX fmax
+#define ORACLE_MAX glsl_max
 #include "tws_oracle_body.h"
 #undef ORACLE_REAL
 #undef ORACLE_SFX
